@@ -1,0 +1,522 @@
+// C ABI of the Green-function operator (include/gila_amd.h): operator state,
+// Fourier Green data upload, workspaces and the matvec launch sequence.
+#include "gila_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gla_launch.hpp"
+#include "gla_setup.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+gla_status fail(gla_status s, const std::string& m)
+{
+    g_err = m;
+    return s;
+}
+
+#define GLA_HIP(call)                                                                             \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(GLA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+struct DevGuard {
+    int prev = -1;
+    explicit DevGuard(int d)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != d) (void)hipSetDevice(d);
+    }
+    ~DevGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Green data and tables on the device; shared between an operator and its
+// adjoint wrapper.
+struct DevData {
+    int device = 0;
+    void* gf = nullptr;
+    void* tab = nullptr;
+    ~DevData()
+    {
+        if (gf || tab) {
+            DevGuard g(device);
+            if (gf) (void)hipFree(gf);
+            if (tab) (void)hipFree(tab);
+        }
+    }
+};
+
+bool vol_eq(const gla_vol& a, const gla_vol& b)
+{
+    for (int d = 0; d < 3; ++d) {
+        if (a.cel[d] != b.cel[d]) return false;
+        if (a.scl_num[d] * b.scl_den[d] != b.scl_num[d] * a.scl_den[d]) return false;
+        if (a.org_num[d] * b.org_den[d] != b.org_num[d] * a.org_den[d]) return false;
+    }
+    return true;
+}
+
+gla_status check_vol(const gla_vol& v, const char* what)
+{
+    for (int d = 0; d < 3; ++d) {
+        if (v.cel[d] < 1) return fail(GLA_ERR_ARG, std::string(what) + ": cell counts must be positive");
+        if (v.scl_den[d] == 0 || v.org_den[d] == 0 || v.scl_num[d] <= 0)
+            return fail(GLA_ERR_ARG, std::string(what) + ": cell size must be a positive rational");
+    }
+    return GLA_OK;
+}
+
+}  // namespace
+
+struct gla_opr_s {
+    gla_opts opts{};
+    gla_vol trg{}, src{};
+    gla_dims dims{};
+    gla::Geom geom{};
+    gla::DevTab tab{};
+    std::shared_ptr<DevData> dev;
+    std::shared_ptr<std::vector<char>> fur_host;   // reference layout, operator dtype
+    void* w1 = nullptr;
+    void* w2 = nullptr;
+    int64_t ws_nrhs = 0;
+    std::mutex mu;
+    ~gla_opr_s()
+    {
+        if (w1 || w2) {
+            DevGuard g(opts.device);
+            if (w1) (void)hipFree(w1);
+            if (w2) (void)hipFree(w2);
+        }
+    }
+};
+
+extern "C" {
+
+const char* gla_last_error(void) { return g_err.c_str(); }
+
+gla_status gla_init(int32_t device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(GLA_ERR_NO_DEVICE, "no HIP device is available");
+    if (device < 0 || device >= n)
+        return fail(GLA_ERR_NO_DEVICE, "HIP device " + std::to_string(device) + " does not exist");
+    return GLA_OK;
+}
+
+gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* out)
+{
+    if (!trg || !out) return fail(GLA_ERR_ARG, "null argument");
+    gla_status s = check_vol(*trg, "target volume");
+    if (s) return s;
+    const bool self = (src == nullptr) || vol_eq(*trg, *src);
+    const gla_vol& sv = self ? *trg : *src;
+    if (!self && (s = check_vol(sv, "source volume"))) return s;
+    gla_dims d{};
+    d.self_opr = self ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        // one source and one target partition: equal cell scales
+        if (trg->scl_num[a] * sv.scl_den[a] != sv.scl_num[a] * trg->scl_den[a]) {
+            const long double r = (long double)trg->scl_num[a] * sv.scl_den[a] /
+                                  ((long double)sv.scl_num[a] * trg->scl_den[a]);
+            const bool integral = std::fabs(r - std::round(r)) < 1e-12L || std::fabs(1 / r - std::round(1 / r)) < 1e-12L;
+            if (!integral) return fail(GLA_ERR_SCALE, "Volume pair must share a common scale grid.");
+            return fail(GLA_ERR_UNSUPPORTED,
+                        "source and target cell sizes differ by an integer factor; the partitioned "
+                        "(multi-resolution) operator is not implemented");
+        }
+        d.trg_cel[a] = trg->cel[a];
+        d.src_cel[a] = sv.cel[a];
+        if ((d.trg_cel[a] + d.src_cel[a]) % 2)
+            return fail(GLA_ERR_SIZE, "the sum of source and target cells must be even on every axis "
+                                      "(regenerate the volumes with an even cell count first)");
+        d.brn[a] = (d.trg_cel[a] + d.src_cel[a]) / 2;
+        const bool same_org = trg->org_num[a] * sv.org_den[a] == sv.org_num[a] * trg->org_den[a];
+        d.sym[a] = (d.trg_cel[a] == d.src_cel[a] && same_org) ? 1 : 0;
+        if (d.sym[a]) {
+            int64_t u = (d.trg_cel[a] + 1) / 2 + (d.trg_cel[a] % 2 == 0);
+            d.fur_ext[a] = u < 2 ? 2 : u;
+        } else {
+            d.fur_ext[a] = d.brn[a];
+        }
+        d.dev_brn[a] = d.brn[a];
+    }
+    *out = d;
+    return GLA_OK;
+}
+
+gla_status gla_opr_get_dims(gla_opr_t h, gla_dims* out)
+{
+    if (!h || !out) return fail(GLA_ERR_ARG, "null argument");
+    *out = h->dims;
+    return GLA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename R>
+gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
+{
+    using C = std::complex<R>;
+    const gla_dims& d = h->dims;
+    const int64_t U0 = d.fur_ext[0], U1 = d.fur_ext[1], U2 = d.fur_ext[2];
+    const int64_t Ut = U0 * U1 * U2;
+    const double scale = 1.0 / (double)(8 * d.brn[0] * d.brn[1] * d.brn[2]);
+    // device layout: ((((b*U2 + j2)*U1 + j1)*U0 + j0)*6 + comp
+    std::vector<C> g((size_t)(8 * Ut * 6));
+    for (int b = 0; b < 8; ++b)
+        for (int c = 0; c < 6; ++c)
+            for (int64_t j = 0; j < Ut; ++j) {
+                const std::complex<double> v = furd[(size_t)((b * 6 + c) * Ut + j)] * scale;
+                g[(size_t)(((int64_t)b * Ut + j) * 6 + c)] = C((R)v.real(), (R)v.imag());
+            }
+    // phase and twiddle tables per axis
+    int64_t ntab = 0;
+    for (int a = 0; a < 3; ++a) ntab += 2 * d.brn[a];
+    std::vector<C> tb((size_t)ntab);
+    int64_t off[3][2];
+    int64_t p = 0;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t B = d.brn[a];
+        off[a][0] = p;
+        for (int64_t n = 0; n < B; ++n) {
+            const double t = -M_PI * (double)n / (double)B;
+            tb[(size_t)(p++)] = C((R)std::cos(t), (R)std::sin(t));
+        }
+        off[a][1] = p;
+        for (int64_t n = 0; n < B; ++n) {
+            const double t = -2.0 * M_PI * (double)n / (double)B;
+            tb[(size_t)(p++)] = C((R)std::cos(t), (R)std::sin(t));
+        }
+    }
+    auto dev = std::make_shared<DevData>();
+    dev->device = h->opts.device;
+    GLA_HIP(hipMalloc(&dev->gf, g.size() * sizeof(C)));
+    GLA_HIP(hipMalloc(&dev->tab, tb.size() * sizeof(C)));
+    GLA_HIP(hipMemcpy(dev->gf, g.data(), g.size() * sizeof(C), hipMemcpyHostToDevice));
+    GLA_HIP(hipMemcpy(dev->tab, tb.data(), tb.size() * sizeof(C), hipMemcpyHostToDevice));
+    h->dev = dev;
+    for (int a = 0; a < 3; ++a) {
+        h->tab.phz[a] = (const C*)dev->tab + off[a][0];
+        h->tab.tw[a] = (const C*)dev->tab + off[a][1];
+        gla::AxisMap& m = h->tab.am[a];
+        m.B = (int)d.brn[a];
+        m.U = (int)d.fur_ext[a];
+        m.sym = (d.sym[a] && d.fur_ext[a] != d.brn[a]) ? 1 : 0;
+        const int T = (int)d.trg_cel[a];
+        m.fe = (T + 1) / 2 + (T % 2 == 0 ? 1 : 0);
+        m.fo = (T + 1) / 2;
+    }
+    h->tab.gf = dev->gf;
+    return GLA_OK;
+}
+
+void fill_geom(gla_opr_t h)
+{
+    gla::Geom& g = h->geom;
+    const gla_dims& d = h->dims;
+    for (int a = 0; a < 3; ++a) {
+        g.S[a] = (int)d.src_cel[a];
+        g.T[a] = (int)d.trg_cel[a];
+        g.B[a] = (int)d.dev_brn[a];
+        g.L[a] = 2 * g.B[a];
+    }
+    g.Y1 = std::max(g.S[1], g.T[1]);
+    g.Z = std::max(g.S[2], g.T[2]);
+    g.xs = (int64_t)g.S[0] * g.S[1] * g.S[2] * 3;
+    g.ys = (int64_t)g.T[0] * g.T[1] * g.T[2] * 3;
+    g.w1s = (int64_t)g.L[0] * g.Y1 * g.Z * 3;
+    g.w2s = (int64_t)g.L[0] * g.L[1] * g.Z * 3;
+}
+
+size_t elem_bytes(int dtype) { return dtype == GLA_C64 ? 8 : 16; }
+
+gla_status ensure_ws(gla_opr_t h, int64_t nrhs)
+{
+    if (h->ws_nrhs >= nrhs) return GLA_OK;
+    const size_t eb = elem_bytes(h->dims.dtype);
+    if (h->w1) { (void)hipFree(h->w1); h->w1 = nullptr; }
+    if (h->w2) { (void)hipFree(h->w2); h->w2 = nullptr; }
+    h->ws_nrhs = 0;
+    GLA_HIP(hipMalloc(&h->w1, (size_t)(h->geom.w1s * nrhs) * eb));
+    GLA_HIP(hipMalloc(&h->w2, (size_t)(h->geom.w2s * nrhs) * eb));
+    h->ws_nrhs = nrhs;
+    return GLA_OK;
+}
+
+gla_status run(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, double ar,
+               double ai, double br, double bi, int axpby, void* stream)
+{
+    if (!h) return fail(GLA_ERR_ARG, "null operator");
+    if (nrhs < 1 || nrhs > 65535) return fail(GLA_ERR_ARG, "nrhs must be in [1, 65535]");
+    if (!x || !y) return fail(GLA_ERR_ARG, "null vector");
+    if (nx != h->geom.xs * nrhs)
+        return fail(GLA_ERR_SIZE, "Size of input vector does not match size of source volume. Required data "
+                                  "layout is (srcCelX, srcCelY, srcCelZ, 3).");
+    if (ny != h->geom.ys * nrhs) return fail(GLA_ERR_SIZE, "Size of output vector does not match the target volume.");
+    std::lock_guard<std::mutex> lk(h->mu);
+    DevGuard g(h->opts.device);
+    gla_status s = ensure_ws(h, nrhs);
+    if (s) return s;
+    hipError_t e;
+    if (h->dims.dtype == GLA_C64)
+        e = gla::run_matvec<float>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby,
+                                   (hipStream_t)stream);
+    else
+        e = gla::run_matvec<double>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby,
+                                    (hipStream_t)stream);
+    if (e != hipSuccess) return fail(GLA_ERR_HIP, std::string("matvec launch: ") + hipGetErrorString(e));
+    return GLA_OK;
+}
+
+template <typename T>
+bool finite_all(const T* p, int64_t n)
+{
+    for (int64_t i = 0; i < 2 * n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, const gla_vol* src,
+                                   const void* fur, int32_t fur_dtype, int64_t fur_len, gla_opr_t* out)
+{
+    if (!opts || !trg || !fur || !out) return fail(GLA_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (opts->dtype != GLA_C64 && opts->dtype != GLA_C128)
+        return fail(GLA_ERR_ARG, "setTyp must be a subtype of Complex (ComplexF32 or ComplexF64)");
+    if (fur_dtype != GLA_C64 && fur_dtype != GLA_C128) return fail(GLA_ERR_ARG, "fur dtype must be c64 or c128");
+    gla_status s = gla_init(opts->device);
+    if (s) return s;
+    gla_dims d;
+    if ((s = gla_opr_dims_for(trg, src, &d))) return s;
+    const int64_t Ut = d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2];
+    if (fur_len != 8 * Ut * 6)
+        return fail(GLA_ERR_SIZE, "Fourier Green data has " + std::to_string(fur_len) + " elements, expected 8*" +
+                                      std::to_string(Ut) + "*6");
+    const bool ok = fur_dtype == GLA_C64 ? finite_all((const float*)fur, fur_len)
+                                         : finite_all((const double*)fur, fur_len);
+    if (!ok) return fail(GLA_ERR_NONFINITE, "Fourier information contains non-numeric values.");
+    for (int a = 0; a < 3; ++a)
+        if (!gla::len_supported((int)d.brn[a]))
+            return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) +
+                                                 " has no compiled transform");
+    d.dtype = opts->dtype;
+    d.adj_mod = opts->adj_mod;
+    std::vector<std::complex<double>> furd((size_t)fur_len);
+    for (int64_t i = 0; i < fur_len; ++i)
+        furd[(size_t)i] = fur_dtype == GLA_C64
+                              ? std::complex<double>(((const float*)fur)[2 * i], ((const float*)fur)[2 * i + 1])
+                              : std::complex<double>(((const double*)fur)[2 * i], ((const double*)fur)[2 * i + 1]);
+    auto h = std::make_unique<gla_opr_s>();
+    h->opts = *opts;
+    h->trg = *trg;
+    h->src = src ? *src : *trg;
+    h->dims = d;
+    fill_geom(h.get());
+    {
+        DevGuard g(opts->device);
+        s = opts->dtype == GLA_C64 ? upload<float>(h.get(), furd) : upload<double>(h.get(), furd);
+        if (s) return s;
+    }
+    // host copy in the operator's element type, rounded as the reference's
+    // setTyp cast rounds it (src/glaMemSup.jl:148)
+    auto fh = std::make_shared<std::vector<char>>((size_t)fur_len * elem_bytes(opts->dtype));
+    for (int64_t i = 0; i < fur_len; ++i) {
+        if (opts->dtype == GLA_C64) {
+            ((float*)fh->data())[2 * i] = (float)furd[(size_t)i].real();
+            ((float*)fh->data())[2 * i + 1] = (float)furd[(size_t)i].imag();
+        } else {
+            ((double*)fh->data())[2 * i] = furd[(size_t)i].real();
+            ((double*)fh->data())[2 * i + 1] = furd[(size_t)i].imag();
+        }
+    }
+    h->fur_host = fh;
+    *out = h.release();
+    return GLA_OK;
+}
+
+gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vol* src, gla_opr_t* out)
+{
+    if (!opts || !trg || !out) return fail(GLA_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (opts->dtype != GLA_C64 && opts->dtype != GLA_C128)
+        return fail(GLA_ERR_ARG, "setTyp must be a subtype of Complex (ComplexF32 or ComplexF64)");
+    gla_status s = gla_init(opts->device);
+    if (s) return s;
+    gla_dims d;
+    if ((s = gla_opr_dims_for(trg, src, &d))) return s;
+    std::vector<std::complex<double>> fur;
+    std::string err;
+    s = gla::setup_fur(*opts, *trg, src ? *src : *trg, d, fur, err);
+    if (s) return fail(s, err);
+    return gla_opr_create_from_fur(opts, trg, src, fur.data(), GLA_C128, (int64_t)fur.size(), out);
+}
+
+gla_status gla_opr_export_fur(gla_opr_t h, void* buf, int64_t len)
+{
+    if (!h || !buf) return fail(GLA_ERR_ARG, "null argument");
+    const int64_t n = (int64_t)(h->fur_host->size() / elem_bytes(h->dims.dtype));
+    if (len < n) return fail(GLA_ERR_SIZE, "export buffer holds " + std::to_string(len) + " elements, need " +
+                                              std::to_string(n));
+    std::memcpy(buf, h->fur_host->data(), h->fur_host->size());
+    return GLA_OK;
+}
+
+gla_status gla_opr_adjoint(gla_opr_t h, gla_opr_t* out)
+{
+    if (!h || !out) return fail(GLA_ERR_ARG, "null argument");
+    auto a = std::make_unique<gla_opr_s>();
+    a->opts = h->opts;
+    a->opts.adj_mod = !h->opts.adj_mod;
+    // target and source volumes swap; the partition info and the Green data
+    // are kept as they are (src/glaLinAlg.jl:16-29)
+    a->trg = h->src;
+    a->src = h->trg;
+    a->dims = h->dims;
+    a->dims.adj_mod = a->opts.adj_mod;
+    a->geom = h->geom;
+    a->tab = h->tab;
+    a->dev = h->dev;
+    a->fur_host = h->fur_host;
+    *out = a.release();
+    return GLA_OK;
+}
+
+gla_status gla_opr_destroy(gla_opr_t h)
+{
+    delete h;
+    return GLA_OK;
+}
+
+gla_status gla_opr_apply(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, void* stream)
+{
+    return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);
+}
+
+gla_status gla_opr_apply_axpby(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
+                               double alpha_re, double alpha_im, double beta_re, double beta_im, void* stream)
+{
+    return run(h, x, nx, y, ny, nrhs, alpha_re, alpha_im, beta_re, beta_im, 1, stream);
+}
+
+gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs)
+{
+    if (!h) return fail(GLA_ERR_ARG, "null operator");
+    DevGuard g(h->opts.device);
+    const size_t eb = elem_bytes(h->dims.dtype);
+    void *dx = nullptr, *dy = nullptr;
+    if (nx != h->geom.xs * nrhs || ny != h->geom.ys * nrhs)
+        return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, nullptr);   // raises the size error
+    GLA_HIP(hipMalloc(&dx, (size_t)nx * eb));
+    hipError_t e = hipMalloc(&dy, (size_t)ny * eb);
+    if (e != hipSuccess) { (void)hipFree(dx); return fail(GLA_ERR_HIP, hipGetErrorString(e)); }
+    gla_status s = GLA_OK;
+    e = hipMemcpy(dx, x, (size_t)nx * eb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        s = run(h, dx, nx, dy, ny, nrhs, 1, 0, 0, 0, 0, nullptr);
+        if (s == GLA_OK) e = hipMemcpy(y, dy, (size_t)ny * eb, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    if (s) return s;
+    if (e != hipSuccess) return fail(GLA_ERR_HIP, hipGetErrorString(e));
+    return GLA_OK;
+}
+
+gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, void* stream,
+                           int32_t reps, double* ms, int64_t* bytes)
+{
+    if (!h || !ms || !bytes || reps < 1) return fail(GLA_ERR_ARG, "bad profile arguments");
+    gla_status s = run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);   // validates, allocates, warms up
+    if (s) return s;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DevGuard g(h->opts.device);
+    const gla::Geom& G = h->geom;
+    const int64_t eb = (int64_t)elem_bytes(h->dims.dtype);
+    const int64_t S = (int64_t)G.S[0] * G.S[1] * G.S[2], T = (int64_t)G.T[0] * G.T[1] * G.T[2];
+    const int64_t a1 = (int64_t)G.L[0] * G.S[1] * G.S[2], a2 = (int64_t)G.L[0] * G.L[1] * G.S[2];
+    const int64_t a3 = (int64_t)G.L[0] * G.L[1] * G.T[2], a4 = (int64_t)G.L[0] * G.T[1] * G.T[2];
+    const int64_t gsz = 8 * h->dims.fur_ext[0] * h->dims.fur_ext[1] * h->dims.fur_ext[2] * 6;
+    const int64_t f = 3 * nrhs * eb;
+    bytes[0] = f * (S + a1);
+    bytes[1] = f * (a1 + a2);
+    bytes[2] = f * (a2 + a3) + gsz * eb;
+    bytes[3] = f * (a3 + a4);
+    bytes[4] = f * (a4 + T);
+    hipEvent_t e0, e1;
+    GLA_HIP(hipEventCreate(&e0));
+    GLA_HIP(hipEventCreate(&e1));
+    hipStream_t st = (hipStream_t)stream;
+    for (int p = 0; p < gla::NPASS; ++p) {
+        hipError_t e = hipSuccess;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && e == hipSuccess; ++r)
+            e = h->dims.dtype == GLA_C64
+                    ? gla::run_pass<float>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, st)
+                    : gla::run_pass<double>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, st);
+        (void)hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return fail(GLA_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
+        }
+        ms[p] = (double)t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // leave y holding a complete matvec again (pass 2 ran in place reps times)
+    return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);
+}
+
+int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs)
+{
+    if (!h) return -1;
+    return (h->geom.w1s + h->geom.w2s) * nrhs * (int64_t)elem_bytes(h->dims.dtype);
+}
+
+}  // extern "C"
+
+namespace gla {
+
+bool len_supported(int n)
+{
+#define GLA_LEN_EQ(k) if (n == k) return true;
+    GLA_FOR_EACH_LEN(GLA_LEN_EQ)
+#undef GLA_LEN_EQ
+    return false;
+}
+
+int len_next_supported(int n)
+{
+    int best = 0;
+#define GLA_LEN_GE(k) if (k >= n && (best == 0 || k < best)) best = k;
+    GLA_FOR_EACH_LEN(GLA_LEN_GE)
+#undef GLA_LEN_GE
+    return best;
+}
+
+}  // namespace gla

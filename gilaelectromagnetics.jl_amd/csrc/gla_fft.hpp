@@ -1,0 +1,368 @@
+// Device FFT engine for the Green-function matvec (gfx950).
+//
+// A "pencil" is one 1-D transform of length N.  TP threads cooperate on a
+// pencil and thread t holds the elements n = t + TP*e, e < E = N/TP, in
+// registers ("natural residue layout").  The transform is a mixed-radix
+// Stockham sequence; stage s with radix r and running span ns reads
+// x[j + q*N/r], multiplies by w^{q*(j mod ns)} (w = e^{-2 pi i/(ns r)} forward),
+// applies DFT_r and writes y[(j/ns)*ns*r + (j mod ns) + q*ns].  When the first
+// radix divides E the first stage reads straight from registers, when the last
+// radix divides E the last stage writes straight back to registers in the same
+// residue layout, so a two-stage transform costs one LDS exchange.  Middle
+// stages go through LDS.  All loops are compile-time unrolled so the register
+// arrays never spill to scratch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gla {
+
+template <typename R>
+struct cpx {
+    R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> mk(R a, R b) { return cpx<R>{a, b}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> operator*(cpx<R> a, cpx<R> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> operator*(R s, cpx<R> a) { return {s * a.x, s * a.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> cmulc(cpx<R> a, cpx<R> b)   // a * conj(b)
+{
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> conjg(cpx<R> a) { return {a.x, -a.y}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename R, bool INV>
+__device__ __forceinline__ cpx<R> mul_mi(cpx<R> a)
+{
+    if (INV) return {-a.y, a.x};
+    return {a.y, -a.x};
+}
+
+// ---------------------------------------------------------------------------
+// compile-time radix plan
+// ---------------------------------------------------------------------------
+struct Plan {
+    int n;
+    int nst;
+    int rad[24];
+};
+
+__host__ __device__ constexpr int smallest_factor(int m)
+{
+    for (int f = 2; f * f <= m; ++f)
+        if (m % f == 0) return f;
+    return m;
+}
+
+// Factor N into radices from {16,8,4,2,3,5,other primes}; the first radix is
+// the largest supported radix dividing both N and E, the last the largest
+// dividing what remains and E, the middle ones anything.
+__host__ __device__ constexpr Plan make_plan(int N, int E)
+{
+    Plan p{N, 0, {}};
+    int m = N;
+    int mid[24] = {};
+    int nmid = 0;
+    int first = 0, last = 0;
+    constexpr int pref[6] = {16, 8, 4, 5, 3, 2};
+    for (int i = 0; i < 6 && !first; ++i)
+        if (m % pref[i] == 0 && E % pref[i] == 0) first = pref[i];
+    if (first) m /= first;
+    for (int i = 0; i < 6 && !last && m > 1; ++i)
+        if (m % pref[i] == 0 && E % pref[i] == 0) last = pref[i];
+    if (last) m /= last;
+    while (m > 1) {
+        int r = 0;
+        for (int i = 0; i < 6 && !r; ++i)
+            if (m % pref[i] == 0) r = pref[i];
+        if (!r) r = smallest_factor(m);
+        mid[nmid++] = r;
+        m /= r;
+    }
+    if (first) p.rad[p.nst++] = first;
+    for (int i = 0; i < nmid; ++i) p.rad[p.nst++] = mid[i];
+    if (last) p.rad[p.nst++] = last;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// radix codelets (in place on a[0..r-1]); w = e^{-+2 pi i / r}
+// ---------------------------------------------------------------------------
+template <typename R, bool INV>
+__device__ __forceinline__ void dft2(cpx<R>* a)
+{
+    cpx<R> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft3(cpx<R>* a)
+{
+    const R c1 = R(-0.5);
+    const R s1 = INV ? R(0.86602540378443864676) : R(-0.86602540378443864676);
+    cpx<R> s = a[1] + a[2], d = a[1] - a[2];
+    cpx<R> u = a[0] + c1 * s;
+    cpx<R> v = {-s1 * d.y, s1 * d.x};
+    a[0] = a[0] + s;
+    a[1] = u + v;
+    a[2] = u - v;
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft4(cpx<R>* a)
+{
+    cpx<R> s02 = a[0] + a[2], d02 = a[0] - a[2];
+    cpx<R> s13 = a[1] + a[3], d13 = mul_mi<R, INV>(a[1] - a[3]);
+    a[0] = s02 + s13;
+    a[1] = d02 + d13;
+    a[2] = s02 - s13;
+    a[3] = d02 - d13;
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft5(cpx<R>* a)
+{
+    const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410);
+    const R s1 = INV ? R(0.95105651629515357212) : R(-0.95105651629515357212);
+    const R s2 = INV ? R(0.58778525229247312917) : R(-0.58778525229247312917);
+    cpx<R> p1 = a[1] + a[4], m1 = a[1] - a[4];
+    cpx<R> p2 = a[2] + a[3], m2 = a[2] - a[3];
+    cpx<R> u1 = a[0] + c1 * p1 + c2 * p2;
+    cpx<R> u2 = a[0] + c2 * p1 + c1 * p2;
+    cpx<R> v1 = s1 * m1 + s2 * m2;          // times i
+    cpx<R> v2 = s2 * m1 - s1 * m2;          // times i
+    a[0] = a[0] + p1 + p2;
+    a[1] = u1 + cpx<R>{-v1.y, v1.x};
+    a[4] = u1 - cpx<R>{-v1.y, v1.x};
+    a[2] = u2 + cpx<R>{-v2.y, v2.x};
+    a[3] = u2 - cpx<R>{-v2.y, v2.x};
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft8(cpx<R>* a)
+{
+    const R h = R(0.70710678118654752440);
+    cpx<R> b[4] = {a[0], a[2], a[4], a[6]};
+    cpx<R> c[4] = {a[1], a[3], a[5], a[7]};
+    dft4<R, INV>(b);
+    dft4<R, INV>(c);
+    // twiddles w8^k, k = 1..3
+    cpx<R> c1 = c[1], c2 = c[2], c3 = c[3];
+    if (!INV) {
+        c[1] = {h * (c1.x + c1.y), h * (c1.y - c1.x)};
+        c[2] = {c2.y, -c2.x};
+        c[3] = {h * (c3.y - c3.x), -h * (c3.x + c3.y)};
+    } else {
+        c[1] = {h * (c1.x - c1.y), h * (c1.x + c1.y)};
+        c[2] = {-c2.y, c2.x};
+        c[3] = {-h * (c3.x + c3.y), h * (c3.x - c3.y)};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = b[k] + c[k];
+        a[k + 4] = b[k] - c[k];
+    }
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void dft16(cpx<R>* a)
+{
+    // 16 = 4 x 4: inner DFT4 over stride-4 elements, twiddle w16^{q1*k2}, outer DFT4
+    const R c8 = R(0.92387953251128675613), s8 = R(0.38268343236508977173), h = R(0.70710678118654752440);
+    cpx<R> b[4][4];
+#pragma unroll
+    for (int q1 = 0; q1 < 4; ++q1) {
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) b[q1][q2] = a[q1 + 4 * q2];
+        dft4<R, INV>(b[q1]);
+    }
+    // twiddle b[q1][k2] *= w16^{q1 k2}
+    const R sg = INV ? R(1) : R(-1);
+    const cpx<R> w[10] = {
+        {R(1), R(0)}, {c8, sg * s8}, {h, sg * h}, {s8, sg * c8}, {R(0), sg},
+        {-s8, sg * c8}, {-h, sg * h}, {-c8, sg * s8}, {R(-1), R(0)}, {-c8, -sg * s8}};
+#pragma unroll
+    for (int q1 = 1; q1 < 4; ++q1)
+#pragma unroll
+        for (int k2 = 1; k2 < 4; ++k2) b[q1][k2] = b[q1][k2] * w[q1 * k2];
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+        cpx<R> c[4] = {b[0][k2], b[1][k2], b[2][k2], b[3][k2]};
+        dft4<R, INV>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) a[k2 + 4 * k1] = c[k1];
+    }
+}
+
+// generic odd radix: direct sum with roots taken from the length-N table
+template <typename R, bool INV, int RAD, int N>
+__device__ __forceinline__ void dftg(cpx<R>* a, const cpx<R>* __restrict__ tw)
+{
+    cpx<R> o[RAD];
+#pragma unroll
+    for (int q = 0; q < RAD; ++q) {
+        cpx<R> acc = a[0];
+#pragma unroll
+        for (int e = 1; e < RAD; ++e) {
+            cpx<R> rt = tw[((e * q) % RAD) * (N / RAD)];
+            acc = acc + (INV ? cmulc(a[e], rt) : a[e] * rt);
+        }
+        o[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < RAD; ++q) a[q] = o[q];
+}
+
+template <typename R, bool INV, int RAD, int N>
+__device__ __forceinline__ void dft(cpx<R>* a, const cpx<R>* __restrict__ tw)
+{
+    if constexpr (RAD == 2) dft2<R, INV>(a);
+    else if constexpr (RAD == 3) dft3<R, INV>(a);
+    else if constexpr (RAD == 4) dft4<R, INV>(a);
+    else if constexpr (RAD == 5) dft5<R, INV>(a);
+    else if constexpr (RAD == 8) dft8<R, INV>(a);
+    else if constexpr (RAD == 16) dft16<R, INV>(a);
+    else dftg<R, INV, RAD, N>(a, tw);
+}
+
+// ---------------------------------------------------------------------------
+// The pencil transform.
+//   v    : E = N/TP register elements, v[e] = x[t + TP*e]
+//   lds  : WG scratch; element pos of this pencil lives at lds[pos*NS + pen*PS]
+//   tw   : table tw[n] = e^{-2 pi i n / N}, n < N (forward sign)
+// All threads of the workgroup must call with the same template arguments
+// (the barriers are workgroup wide).  `act` false masks a pencil beyond the
+// end of the data: its threads still take part in every barrier.
+// ---------------------------------------------------------------------------
+template <typename R, int N, int TP, int NS, int PS, bool INV>
+struct PencilFft {
+    static constexpr int E = N / TP;
+    static_assert(N % TP == 0, "TP must divide N");
+    static constexpr Plan P = make_plan(N, E);
+    static constexpr bool FIRST_REG = (P.nst > 0) && (E % P.rad[0] == 0);
+    static constexpr bool LAST_REG = (P.nst > 1) && (E % P.rad[P.nst - 1] == 0);
+
+    // one stage reading LDS and writing LDS (any thread mapping)
+    template <int S>
+    static __device__ __forceinline__ void stage_lds(cpx<R>* lds, int t, int pen, const cpx<R>* tw)
+    {
+        constexpr int RAD = P.rad[S];
+        constexpr int NB = N / RAD;                      // butterflies
+        constexpr int NU = (NB + TP - 1) / TP;          // per thread (upper bound)
+        constexpr int NSP = [] { int ns = 1; for (int i = 0; i < S; ++i) ns *= P.rad[i]; return ns; }();
+        cpx<R> a[NU][RAD];
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int j = t + TP * u;
+            if (NB % TP == 0 || j < NB) {
+#pragma unroll
+                for (int q = 0; q < RAD; ++q) a[u][q] = lds[(j + q * NB) * NS + pen * PS];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int j = t + TP * u;
+            if (NB % TP == 0 || j < NB) {
+                const int k = j % NSP;
+                if constexpr (NSP > 1) {
+#pragma unroll
+                    for (int q = 1; q < RAD; ++q) {
+                        cpx<R> w = tw[(q * k * (N / (NSP * RAD))) % N];
+                        a[u][q] = INV ? cmulc(a[u][q], w) : a[u][q] * w;
+                    }
+                }
+                dft<R, INV, RAD, N>(a[u], tw);
+                const int base = (j / NSP) * NSP * RAD + k;
+#pragma unroll
+                for (int q = 0; q < RAD; ++q) lds[(base + q * NSP) * NS + pen * PS] = a[u][q];
+            }
+        }
+        __syncthreads();
+    }
+
+    template <int S>
+    static __device__ __forceinline__ void mid_stages(cpx<R>* lds, int t, int pen, const cpx<R>* tw)
+    {
+        constexpr int LASTS = LAST_REG ? P.nst - 1 : P.nst;
+        if constexpr (S < LASTS) {
+            stage_lds<S>(lds, t, pen, tw);
+            mid_stages<S + 1>(lds, t, pen, tw);
+        }
+    }
+
+    static __device__ __forceinline__ void run(cpx<R> (&v)[E], cpx<R>* lds, int t, int pen,
+                                               const cpx<R>* __restrict__ tw)
+    {
+        if constexpr (P.nst == 0) {
+            return;
+        } else if constexpr (P.nst == 1 && FIRST_REG) {
+            // whole transform inside one thread (E == N)
+            dft<R, INV, P.rad[0], N>(v, tw);
+        } else {
+            // ---- first stage
+            if constexpr (FIRST_REG) {
+                constexpr int RAD = P.rad[0];
+                constexpr int EU = E / RAD;   // butterflies per thread
+#pragma unroll
+                for (int u = 0; u < EU; ++u) {
+                    cpx<R> a[RAD];
+#pragma unroll
+                    for (int q = 0; q < RAD; ++q) a[q] = v[u + q * EU];
+                    dft<R, INV, RAD, N>(a, tw);
+                    const int j = t + TP * u;    // butterfly index, span 1
+#pragma unroll
+                    for (int q = 0; q < RAD; ++q) lds[(j * RAD + q) * NS + pen * PS] = a[q];
+                }
+                __syncthreads();
+                mid_stages<1>(lds, t, pen, tw);
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) lds[(t + TP * e) * NS + pen * PS] = v[e];
+                __syncthreads();
+                mid_stages<0>(lds, t, pen, tw);
+            }
+            // ---- last stage
+            if constexpr (LAST_REG) {
+                constexpr int S = P.nst - 1;
+                constexpr int RAD = P.rad[S];
+                constexpr int EU = E / RAD;
+                constexpr int NSP = N / RAD;
+#pragma unroll
+                for (int u = 0; u < EU; ++u) {
+                    const int j = t + TP * u;
+                    cpx<R> a[RAD];
+#pragma unroll
+                    for (int q = 0; q < RAD; ++q) a[q] = lds[(j + q * NSP) * NS + pen * PS];
+#pragma unroll
+                    for (int q = 1; q < RAD; ++q) {
+                        cpx<R> w = tw[(q * j) % N];
+                        a[q] = INV ? cmulc(a[q], w) : a[q] * w;
+                    }
+                    dft<R, INV, RAD, N>(a, tw);
+#pragma unroll
+                    for (int q = 0; q < RAD; ++q) v[u + q * EU] = a[q];
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) v[e] = lds[(t + TP * e) * NS + pen * PS];
+            }
+            __syncthreads();   // LDS free for the next transform
+        }
+    }
+};
+
+}  // namespace gla

@@ -1,0 +1,358 @@
+// Fused pass kernels of the Green-function matvec y = P IDFT_L(G^ . DFT_L(E x)).
+//
+// Every transformed axis d uses the even/odd frequency split of the length
+// L_d = 2 B_d embedding (the pruned first radix-2 stage): with x[n] := 0 for
+// n >= S_d,
+//   even[n] = x[n] + x[n+B],  odd[n] = phi[n] (x[n] - x[n+B]),  phi[n] = e^{-i pi n/B},
+// and DFT_L(E x)[2k+p] = DFT_B(p ? odd : even)[k].  The inverse merges
+//   y[n] = e[n] + conj(phi[n]) o[n]      (n < min(T,B)),
+//   y[n+B] = e[n] - conj(phi[n]) o[n]    (n < T-B),
+// and all scale factors (1/L per axis) are folded into the stored G^.
+//
+// Workspace layouts (column-major, axis 1 fastest; index p*B+k on a
+// transformed axis, p the parity):
+//   X  (S1, S2, S3, 3)            source field
+//   W1 (L1, Y1, Z, 3)             after pass 1 / before pass 5, Y1 = max(S2,T2)
+//   W2 (L1, L2, Z, 3)             after pass 2 / pass 3, Z = max(S3,T3)
+//   Y  (T1, T2, T3, 3)            target field
+// Passes: 1 x-forward, 2 y-forward, 3 z-forward * G^ * z-inverse (in place),
+// 4 y-inverse, 5 x-inverse (+ alpha/beta epilogue).
+#pragma once
+#include "gla_fft.hpp"
+
+namespace gla {
+
+// per-axis description of how a branch frequency k maps onto stored G^
+// entries (restating mulBrnHst!, src/glaActSup.jl:691-753): on a symmetric
+// axis the even branch reads k for k < fe and B-k (sign -1 on off-diagonal
+// components involving this axis) beyond; the odd branch reads k for k < fo
+// and B-1-k beyond.
+struct AxisMap {
+    int B, U, sym, fe, fo;
+};
+
+__device__ __forceinline__ void axis_lookup(const AxisMap& a, int p, int k, int& j, int& s)
+{
+    if (!a.sym) { j = k; s = 1; return; }
+    if (p == 0) {
+        if (k < a.fe) { j = k; s = 1; } else { j = a.B - k; s = -1; }
+    } else {
+        if (k < a.fo) { j = k; s = 1; } else { j = a.B - 1 - k; s = -1; }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Pass 1: x-forward on contiguous rows.  One pencil = one (i2,i3,c) row of X.
+// --------------------------------------------------------------------------
+template <typename R, int N, int TP, int PEN>
+__global__ __launch_bounds__(TP * PEN) void k_fwd_x(const cpx<R>* __restrict__ X, cpx<R>* __restrict__ W1,
+                                                    int S1, int S2, int S3, int Y1, int Z,
+                                                    int64_t xs, int64_t ws,
+                                                    const cpx<R>* __restrict__ phz,
+                                                    const cpx<R>* __restrict__ tw)
+{
+    constexpr int E = N / TP;
+    constexpr int PS = N + 1;
+    __shared__ cpx<R> lds[PEN * PS];
+    const int t = threadIdx.x % TP, pen = threadIdx.x / TP;
+    const int rows = S2 * S3 * 3;
+    const int row = blockIdx.x * PEN + pen;
+    const bool act = row < rows;
+    const int i2 = act ? row % S2 : 0, i3 = act ? (row / S2) % S3 : 0, c = act ? row / (S2 * S3) : 0;
+    const cpx<R>* xr = X + (int64_t)blockIdx.y * xs + (int64_t)row * S1;
+    cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
+    cpx<R> ve[E], vo[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + TP * e;
+        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+        if (act && n < S1) a = xr[n];
+        if (act && n + N < S1) b = xr[n + N];
+        ve[e] = a + b;
+        vo[e] = phz[n] * (a - b);
+    }
+    PencilFft<R, N, TP, 1, PS, false>::run(ve, lds, t, pen, tw);
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) wr[t + TP * e] = ve[e];
+    }
+    PencilFft<R, N, TP, 1, PS, false>::run(vo, lds, t, pen, tw);
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) wr[N + t + TP * e] = vo[e];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Pass 5: x-inverse + merge/crop onto T1, optional y = alpha*res + beta*y.
+// --------------------------------------------------------------------------
+template <typename R, int N, int TP, int PEN>
+__global__ __launch_bounds__(TP * PEN) void k_inv_x(const cpx<R>* __restrict__ W1, cpx<R>* __restrict__ Y,
+                                                    int T1, int T2, int T3, int Y1, int Z,
+                                                    int64_t ws, int64_t ys,
+                                                    const cpx<R>* __restrict__ phz,
+                                                    const cpx<R>* __restrict__ tw,
+                                                    cpx<R> alpha, cpx<R> beta, int axpby)
+{
+    constexpr int E = N / TP;
+    constexpr int PS = N + 1;
+    __shared__ cpx<R> lds[PEN * PS];
+    const int t = threadIdx.x % TP, pen = threadIdx.x / TP;
+    const int rows = T2 * T3 * 3;
+    const int row = blockIdx.x * PEN + pen;
+    const bool act = row < rows;
+    const int i2 = act ? row % T2 : 0, i3 = act ? (row / T2) % T3 : 0, c = act ? row / (T2 * T3) : 0;
+    const cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
+    cpx<R>* yr = Y + (int64_t)blockIdx.y * ys + (int64_t)row * T1;
+    cpx<R> ve[E], vo[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = t + TP * e;
+        ve[e] = act ? wr[k] : cpx<R>{R(0), R(0)};
+        vo[e] = act ? wr[N + k] : cpx<R>{R(0), R(0)};
+    }
+    PencilFft<R, N, TP, 1, PS, true>::run(ve, lds, t, pen, tw);
+    PencilFft<R, N, TP, 1, PS, true>::run(vo, lds, t, pen, tw);
+    if (!act) return;
+    const int top = T1 < N ? T1 : N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + TP * e;
+        const cpx<R> po = cmulc(vo[e], phz[n]);
+        if (n < top) {
+            cpx<R> r = ve[e] + po;
+            if (axpby) r = alpha * r + (beta.x == R(0) && beta.y == R(0) ? cpx<R>{R(0), R(0)} : beta * yr[n]);
+            yr[n] = r;
+        }
+        if (n + N < T1) {
+            cpx<R> r = ve[e] - po;
+            if (axpby) r = alpha * r + (beta.x == R(0) && beta.y == R(0) ? cpx<R>{R(0), R(0)} : beta * yr[n + N]);
+            yr[n + N] = r;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Pass 2: y-forward on bundles of PX consecutive axis-1 columns.
+// --------------------------------------------------------------------------
+template <typename R, int N, int TP, int PX>
+__global__ __launch_bounds__(TP * PX) void k_fwd_y(const cpx<R>* __restrict__ W1, cpx<R>* __restrict__ W2,
+                                                   int L1, int S2, int S3, int Y1, int Z, int64_t w1s,
+                                                   int64_t w2s, const cpx<R>* __restrict__ phz,
+                                                   const cpx<R>* __restrict__ tw)
+{
+    constexpr int E = N / TP;
+    __shared__ cpx<R> lds[PX * N];
+    const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
+    const int m1 = blockIdx.x * PX + xl;
+    const bool act = m1 < L1;
+    const int i3 = blockIdx.y % S3, c = blockIdx.y / S3;
+    const cpx<R>* src = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + m1;
+    cpx<R>* dst = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + m1;
+    cpx<R> ve[E], vo[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + TP * e;
+        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+        if (act && n < S2) a = src[(int64_t)n * L1];
+        if (act && n + N < S2) b = src[(int64_t)(n + N) * L1];
+        ve[e] = a + b;
+        vo[e] = phz[n] * (a - b);
+    }
+    PencilFft<R, N, TP, PX, 1, false>::run(ve, lds, t, xl, tw);
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[(int64_t)(t + TP * e) * L1] = ve[e];
+    }
+    PencilFft<R, N, TP, PX, 1, false>::run(vo, lds, t, xl, tw);
+    if (act) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) dst[(int64_t)(N + t + TP * e) * L1] = vo[e];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Pass 4: y-inverse + merge/crop onto T2.
+// --------------------------------------------------------------------------
+template <typename R, int N, int TP, int PX>
+__global__ __launch_bounds__(TP * PX) void k_inv_y(const cpx<R>* __restrict__ W2, cpx<R>* __restrict__ W1,
+                                                   int L1, int T2, int T3, int Y1, int Z, int64_t w2s,
+                                                   int64_t w1s, const cpx<R>* __restrict__ phz,
+                                                   const cpx<R>* __restrict__ tw)
+{
+    constexpr int E = N / TP;
+    __shared__ cpx<R> lds[PX * N];
+    const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
+    const int m1 = blockIdx.x * PX + xl;
+    const bool act = m1 < L1;
+    const int i3 = blockIdx.y % T3, c = blockIdx.y / T3;
+    const cpx<R>* src = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + m1;
+    cpx<R>* dst = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + m1;
+    cpx<R> ve[E], vo[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k = t + TP * e;
+        ve[e] = act ? src[(int64_t)k * L1] : cpx<R>{R(0), R(0)};
+        vo[e] = act ? src[(int64_t)(N + k) * L1] : cpx<R>{R(0), R(0)};
+    }
+    PencilFft<R, N, TP, PX, 1, true>::run(ve, lds, t, xl, tw);
+    PencilFft<R, N, TP, PX, 1, true>::run(vo, lds, t, xl, tw);
+    if (!act) return;
+    const int top = T2 < N ? T2 : N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + TP * e;
+        const cpx<R> po = cmulc(vo[e], phz[n]);
+        if (n < top) dst[(int64_t)n * L1] = ve[e] + po;
+        if (n + N < T2) dst[(int64_t)(n + N) * L1] = ve[e] - po;
+    }
+}
+
+// --------------------------------------------------------------------------
+// Pass 3: z-forward, G^ multiply, z-inverse, merge/crop onto T3 -- in place
+// on W2 (each workgroup owns its columns: every load precedes the first
+// barrier, every store follows the last).
+// MODE 0: S3 == T3 == B3; MODE 1: S3 > B3 (spill-over add); MODE 2: T3 > B3.
+// G^ device layout: gf[(((b*U3 + j3)*U2 + j2)*U1 + j1)*6 + comp],
+// comp order [xx,yy,zz,xy,xz,yz], pre-scaled by 1/(L1 L2 L3).
+// --------------------------------------------------------------------------
+template <typename R, int N, int TP, int PX, int MODE>
+__global__ __launch_bounds__(TP * PX) void k_zpass(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int T3,
+                                                   int Z, int64_t w2s, const cpx<R>* __restrict__ gf,
+                                                   AxisMap a1, AxisMap a2, AxisMap a3,
+                                                   const cpx<R>* __restrict__ phz,
+                                                   const cpx<R>* __restrict__ tw)
+{
+    constexpr int E = N / TP;
+    __shared__ cpx<R> lds[PX * N];
+    const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
+    const int m1 = blockIdx.x * PX + xl;
+    const int m2 = blockIdx.y;
+    const bool act = m1 < L1;
+    const int64_t cs = (int64_t)Z * L2 * L1;      // component stride
+    cpx<R>* col = W2 + (int64_t)blockIdx.z * w2s + (int64_t)m2 * L1 + m1;
+    const int64_t zs = (int64_t)L2 * L1;
+
+    cpx<R> xa[3][E];
+    cpx<R> xb[MODE == 1 ? 3 : 1][MODE == 1 ? E : 1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int n = t + TP * e;
+            xa[c][e] = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
+            if constexpr (MODE == 1)
+                xb[c][e] = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
+        }
+
+    // per-column branch parities and stored indices on axes 1 and 2
+    const int p1 = m1 >= a1.B ? 1 : 0, k1 = m1 - p1 * a1.B;
+    const int p2 = m2 >= a2.B ? 1 : 0, k2 = m2 - p2 * a2.B;
+    int j1, s1, j2, s2;
+    axis_lookup(a1, p1, act ? k1 : 0, j1, s1);
+    axis_lookup(a2, p2, k2, j2, s2);
+    const R s12 = R(s1 * s2);
+
+    cpx<R> acc[3][E];
+    cpx<R> accb[MODE == 2 ? 3 : 1][MODE == 2 ? E : 1];
+#pragma unroll
+    for (int p3 = 0; p3 < 2; ++p3) {
+        cpx<R> w[3][E];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int n = t + TP * e;
+                cpx<R> a = xa[c][e];
+                cpx<R> b = {R(0), R(0)};
+                if constexpr (MODE == 1) b = xb[c][e];
+                w[c][e] = p3 == 0 ? a + b : phz[n] * (a - b);
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, false>::run(w[c], lds, t, xl, tw);
+        // symmetric 3x3 multiply (mulKerHst!, src/glaActSup.jl:879-899)
+        const int b = (p1 << 2) | (p2 << 1) | p3;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int k3 = t + TP * e;
+            int j3, s3;
+            axis_lookup(a3, p3, k3, j3, s3);
+            if (act) {
+                const cpx<R>* g = gf + ((((int64_t)b * a3.U + j3) * a2.U + j2) * a1.U + j1) * 6;
+                const cpx<R> gxx = g[0], gyy = g[1], gzz = g[2];
+                const R s13 = R(s1 * s3), s23 = R(s2 * s3);
+                const cpx<R> gxy = s12 * g[3], gxz = s13 * g[4], gyz = s23 * g[5];
+                const cpx<R> x0 = w[0][e], x1 = w[1][e], x2 = w[2][e];
+                w[0][e] = gxx * x0 + gxy * x1 + gxz * x2;
+                w[1][e] = gxy * x0 + gyy * x1 + gyz * x2;
+                w[2][e] = gxz * x0 + gyz * x1 + gzz * x2;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, true>::run(w[c], lds, t, xl, tw);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (p3 == 0) {
+                    acc[c][e] = w[c][e];
+                    if constexpr (MODE == 2) accb[c][e] = w[c][e];
+                } else {
+                    const int n = t + TP * e;
+                    const cpx<R> po = cmulc(w[c][e], phz[n]);
+                    acc[c][e] = acc[c][e] + po;
+                    if constexpr (MODE == 2) accb[c][e] = accb[c][e] - po;
+                }
+            }
+    }
+    if (!act) return;
+    const int top = T3 < N ? T3 : N;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int n = t + TP * e;
+            if (n < top) col[c * cs + (int64_t)n * zs] = acc[c][e];
+            if constexpr (MODE == 2)
+                if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = accb[c][e];
+        }
+}
+
+// --------------------------------------------------------------------------
+// Forward z pass without the multiply (operator setup): split S3 -> 2 x B3
+// and transform, in place on W2 (L1, L2, Z, 3) per right-hand side.
+// --------------------------------------------------------------------------
+template <typename R, int N, int TP, int PX>
+__global__ __launch_bounds__(TP * PX) void k_fwd_z(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int Z,
+                                                   int64_t w2s, const cpx<R>* __restrict__ phz,
+                                                   const cpx<R>* __restrict__ tw)
+{
+    constexpr int E = N / TP;
+    __shared__ cpx<R> lds[PX * N];
+    const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
+    const int m1 = blockIdx.x * PX + xl;
+    const bool act = m1 < L1;
+    const int m2 = blockIdx.y % L2, c = blockIdx.y / L2;
+    const int64_t zs = (int64_t)L2 * L1;
+    cpx<R>* col = W2 + (int64_t)blockIdx.z * w2s + (int64_t)c * Z * zs + (int64_t)m2 * L1 + m1;
+    cpx<R> ve[E], vo[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int n = t + TP * e;
+        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+        if (act && n < S3) a = col[(int64_t)n * zs];
+        if (act && n + N < S3) b = col[(int64_t)(n + N) * zs];
+        ve[e] = a + b;
+        vo[e] = phz[n] * (a - b);
+    }
+    PencilFft<R, N, TP, PX, 1, false>::run(ve, lds, t, xl, tw);
+    PencilFft<R, N, TP, PX, 1, false>::run(vo, lds, t, xl, tw);
+    if (!act) return;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        col[(int64_t)(t + TP * e) * zs] = ve[e];
+        col[(int64_t)(N + t + TP * e) * zs] = vo[e];
+    }
+}
+
+}  // namespace gla

@@ -1,0 +1,51 @@
+// Launch layer between the C-ABI (gla_capi.cpp) and the pass kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "gla_kernels.hpp"
+
+// Branch lengths B (per-axis FFT lengths) compiled into the device library.
+// Other lengths are reached by embedding into the next listed length (the
+// circulant only needs L >= S + T - 1).
+#define GLA_FOR_EACH_LEN(X)                                                                   \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) \
+        X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512)
+
+namespace gla {
+
+struct Geom {
+    int S[3], T[3], B[3], L[3];
+    int Y1, Z;             // padded workspace extents: max(S2,T2), max(S3,T3)
+    int64_t xs, ys;        // per-RHS strides of X and Y (elements)
+    int64_t w1s, w2s;      // per-RHS strides of W1 and W2 (elements)
+};
+
+struct DevTab {
+    const void* phz[3];    // phi_d[n] = e^{-i pi n / B_d}, n < B_d
+    const void* tw[3];     // e^{-2 pi i n / B_d}, n < B_d
+    AxisMap am[3];
+    const void* gf;        // G^ in device layout
+};
+
+bool len_supported(int n);
+int len_next_supported(int n);   // smallest supported length >= n, or 0
+
+constexpr int NPASS = 5;
+
+// one pass of the pipeline: 0 x-forward, 1 y-forward, 2 z fused, 3 y-inverse,
+// 4 x-inverse
+template <typename R>
+hipError_t run_pass(int pass, const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2,
+                    int nrhs, double ar, double ai, double br, double bi, int axpby, hipStream_t st);
+
+// forward x, y, z passes only (operator setup): X (S1,S2,S3,3) -> W2 with
+// every axis split and transformed
+template <typename R>
+hipError_t run_fwd3(const Geom& g, const DevTab& tb, const void* X, void* W1, void* W2, int nrhs, hipStream_t st);
+
+template <typename R>
+hipError_t run_matvec(const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2,
+                      int nrhs, double ar, double ai, double br, double bi, int axpby,
+                      hipStream_t st);
+
+}  // namespace gla

@@ -1,0 +1,174 @@
+// Per-precision instantiation of the pass kernels; included by
+// gla_c64.hip (R = float) and gla_c128.hip (R = double).
+#pragma once
+#include "gla_launch.hpp"
+
+namespace gla {
+
+// elements per thread: the divisor of N closest to `target` from below,
+// raised until the workgroup fits in `maxtp` threads per pencil
+__host__ __device__ constexpr int pick_e(int N, int target, int maxtp)
+{
+    int e = 1;
+    for (int d = 1; d <= N && d <= target; ++d)
+        if (N % d == 0) e = d;
+    while (N / e > maxtp) {
+        int nxt = e + 1;
+        while (N % nxt) ++nxt;
+        e = nxt;
+    }
+    return e;
+}
+
+template <typename R>
+struct Cfg {
+    static constexpr bool DBL = sizeof(R) == 8;
+    static constexpr int PX = DBL ? 8 : 16;           // 128-byte rows across a bundle
+    static constexpr int E_ROW = 16;                  // passes 1, 2, 4, 5
+    static constexpr int E_Z = DBL ? 4 : 8;           // pass 3 (3 components live)
+};
+
+template <typename R, int N>
+struct LenK {
+    using C = Cfg<R>;
+    static constexpr int TP_ROW = N / pick_e(N, C::E_ROW, 256);
+    static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
+    static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / C::PX);
+    static constexpr int TP_Z = N / pick_e(N, C::E_Z, 1024 / C::PX);
+};
+
+template <typename R, int N>
+static hipError_t launch_fwd_x(const Geom& g, const DevTab& tb, const void* X, void* W1, int nrhs, hipStream_t st)
+{
+    using K = LenK<R, N>;
+    const int rows = g.S[1] * g.S[2] * 3;
+    dim3 grid((rows + K::PEN - 1) / K::PEN, nrhs);
+    hipLaunchKernelGGL((k_fwd_x<R, N, K::TP_ROW, K::PEN>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
+                       (const cpx<R>*)X, (cpx<R>*)W1, g.S[0], g.S[1], g.S[2], g.Y1, g.Z, g.xs, g.w1s,
+                       (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0]);
+    return hipGetLastError();
+}
+
+template <typename R, int N>
+static hipError_t launch_inv_x(const Geom& g, const DevTab& tb, const void* W1, void* Y, int nrhs,
+                               cpx<R> al, cpx<R> be, int axpby, hipStream_t st)
+{
+    using K = LenK<R, N>;
+    const int rows = g.T[1] * g.T[2] * 3;
+    dim3 grid((rows + K::PEN - 1) / K::PEN, nrhs);
+    hipLaunchKernelGGL((k_inv_x<R, N, K::TP_ROW, K::PEN>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
+                       (const cpx<R>*)W1, (cpx<R>*)Y, g.T[0], g.T[1], g.T[2], g.Y1, g.Z, g.w1s, g.ys,
+                       (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0], al, be, axpby);
+    return hipGetLastError();
+}
+
+template <typename R, int N>
+static hipError_t launch_fwd_y(const Geom& g, const DevTab& tb, const void* W1, void* W2, int nrhs, hipStream_t st)
+{
+    using K = LenK<R, N>;
+    constexpr int PX = Cfg<R>::PX;
+    dim3 grid((g.L[0] + PX - 1) / PX, g.S[2] * 3, nrhs);
+    hipLaunchKernelGGL((k_fwd_y<R, N, K::TP_Y, PX>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W1,
+                       (cpx<R>*)W2, g.L[0], g.S[1], g.S[2], g.Y1, g.Z, g.w1s, g.w2s,
+                       (const cpx<R>*)tb.phz[1], (const cpx<R>*)tb.tw[1]);
+    return hipGetLastError();
+}
+
+template <typename R, int N>
+static hipError_t launch_inv_y(const Geom& g, const DevTab& tb, const void* W2, void* W1, int nrhs, hipStream_t st)
+{
+    using K = LenK<R, N>;
+    constexpr int PX = Cfg<R>::PX;
+    dim3 grid((g.L[0] + PX - 1) / PX, g.T[2] * 3, nrhs);
+    hipLaunchKernelGGL((k_inv_y<R, N, K::TP_Y, PX>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W2,
+                       (cpx<R>*)W1, g.L[0], g.T[1], g.T[2], g.Y1, g.Z, g.w2s, g.w1s,
+                       (const cpx<R>*)tb.phz[1], (const cpx<R>*)tb.tw[1]);
+    return hipGetLastError();
+}
+
+template <typename R, int N>
+static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
+{
+    using K = LenK<R, N>;
+    constexpr int PX = Cfg<R>::PX;
+    dim3 grid((g.L[0] + PX - 1) / PX, g.L[1], nrhs);
+    dim3 blk(K::TP_Z * PX);
+    const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);
+#define GLA_ZARGS                                                                                   \
+    (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
+        tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2]
+    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0>), grid, blk, 0, st, GLA_ZARGS);
+    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1>), grid, blk, 0, st, GLA_ZARGS);
+    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2>), grid, blk, 0, st, GLA_ZARGS);
+#undef GLA_ZARGS
+    return hipGetLastError();
+}
+
+template <typename R, int N>
+static hipError_t launch_fwd_z(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
+{
+    using K = LenK<R, N>;
+    constexpr int PX = Cfg<R>::PX;
+    dim3 grid((g.L[0] + PX - 1) / PX, g.L[1] * 3, nrhs);
+    hipLaunchKernelGGL((k_fwd_z<R, N, K::TP_Y, PX>), grid, dim3(K::TP_Y * PX), 0, st, (cpx<R>*)W2, g.L[0], g.L[1],
+                       g.S[2], g.Z, g.w2s, (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2]);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t run_fwd3(const Geom& g, const DevTab& tb, const void* X, void* W1, void* W2, int nrhs, hipStream_t st)
+{
+    hipError_t e = hipErrorInvalidValue;
+#define GLA_CASE_1(n) case n: e = launch_fwd_x<R, n>(g, tb, X, W1, nrhs, st); break;
+#define GLA_CASE_2(n) case n: e = launch_fwd_y<R, n>(g, tb, W1, W2, nrhs, st); break;
+#define GLA_CASE_3(n) case n: e = launch_fwd_z<R, n>(g, tb, W2, nrhs, st); break;
+    switch (g.B[0]) { GLA_FOR_EACH_LEN(GLA_CASE_1) default: return hipErrorInvalidValue; }
+    if (e != hipSuccess) return e;
+    switch (g.B[1]) { GLA_FOR_EACH_LEN(GLA_CASE_2) default: return hipErrorInvalidValue; }
+    if (e != hipSuccess) return e;
+    switch (g.B[2]) { GLA_FOR_EACH_LEN(GLA_CASE_3) default: return hipErrorInvalidValue; }
+#undef GLA_CASE_1
+#undef GLA_CASE_2
+#undef GLA_CASE_3
+    return e;
+}
+
+template <typename R>
+hipError_t run_pass(int pass, const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2,
+                    int nrhs, double ar, double ai, double br, double bi, int axpby, hipStream_t st)
+{
+    hipError_t e = hipErrorInvalidValue;
+    const cpx<R> al{R(ar), R(ai)}, be{R(br), R(bi)};
+#define GLA_CASE_1(n) case n: e = launch_fwd_x<R, n>(g, tb, X, W1, nrhs, st); break;
+#define GLA_CASE_2(n) case n: e = launch_fwd_y<R, n>(g, tb, W1, W2, nrhs, st); break;
+#define GLA_CASE_3(n) case n: e = launch_zpass<R, n>(g, tb, W2, nrhs, st); break;
+#define GLA_CASE_4(n) case n: e = launch_inv_y<R, n>(g, tb, W2, W1, nrhs, st); break;
+#define GLA_CASE_5(n) case n: e = launch_inv_x<R, n>(g, tb, W1, Y, nrhs, al, be, axpby, st); break;
+    switch (pass) {
+        case 0: switch (g.B[0]) { GLA_FOR_EACH_LEN(GLA_CASE_1) default: break; } break;
+        case 1: switch (g.B[1]) { GLA_FOR_EACH_LEN(GLA_CASE_2) default: break; } break;
+        case 2: switch (g.B[2]) { GLA_FOR_EACH_LEN(GLA_CASE_3) default: break; } break;
+        case 3: switch (g.B[1]) { GLA_FOR_EACH_LEN(GLA_CASE_4) default: break; } break;
+        case 4: switch (g.B[0]) { GLA_FOR_EACH_LEN(GLA_CASE_5) default: break; } break;
+        default: break;
+    }
+#undef GLA_CASE_1
+#undef GLA_CASE_2
+#undef GLA_CASE_3
+#undef GLA_CASE_4
+#undef GLA_CASE_5
+    return e;
+}
+
+template <typename R>
+hipError_t run_matvec(const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2, int nrhs,
+                      double ar, double ai, double br, double bi, int axpby, hipStream_t st)
+{
+    for (int p = 0; p < NPASS; ++p) {
+        hipError_t e = run_pass<R>(p, g, tb, X, Y, W1, W2, nrhs, ar, ai, br, bi, axpby, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace gla

@@ -1,0 +1,174 @@
+/*
+ * gila_amd.h -- C ABI of the MI355X Green-function operator library
+ * (libgila_amd.so).  Plain pointers and sizes only.
+ *
+ * Arrays use the reference's Julia layout: column-major, axis 1 fastest,
+ * then axis 2, axis 3, field component (1..3), then right-hand side, with
+ * interleaved (re, im) complex values that are layout-compatible with Julia
+ * ComplexF32 / ComplexF64 and C float/double _Complex.
+ *
+ * Citations are to the reference repository GilaElectromagnetics.jl.
+ *
+ * Error convention: every call returns a gla_status; GLA_OK is 0.  The text
+ * of the last error raised on the calling thread is returned by
+ * gla_last_error().  One status per row of the reference's error table:
+ *   GLA_ERR_SIZE        wrong input size           (src/glaAct.jl:45, ErrorException)
+ *   GLA_ERR_NONFINITE   NaN/Inf in the Green data  (src/glaMemSup.jl:94,111,162)
+ *   GLA_ERR_NO_DEVICE   no usable GPU              (src/glaMemSup.jl:23)
+ *   GLA_ERR_OVERLAP     overlapping volumes        (src/glaGen.jl:78)
+ *   GLA_ERR_SCALE       incompatible cell scales   (src/glaMem.jl:141,194,208)
+ *   GLA_ERR_TYPE        element-type mismatch      (src/glaLinAlg.jl:35, AssertionError)
+ *   GLA_ERR_ARG         invalid argument / non-Complex set type (src/glaMemSup.jl:311,342)
+ *   GLA_ERR_UNSUPPORTED a configuration this library does not implement
+ *   GLA_ERR_HIP         a HIP runtime failure (out of memory, launch failure)
+ */
+#ifndef GILA_AMD_H
+#define GILA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GLA_OK = 0,
+    GLA_ERR_SIZE = 1,
+    GLA_ERR_NONFINITE = 2,
+    GLA_ERR_NO_DEVICE = 3,
+    GLA_ERR_OVERLAP = 4,
+    GLA_ERR_SCALE = 5,
+    GLA_ERR_TYPE = 6,
+    GLA_ERR_ARG = 7,
+    GLA_ERR_UNSUPPORTED = 8,
+    GLA_ERR_HIP = 9
+} gla_status;
+
+/* element types: the reference's setTyp */
+typedef enum { GLA_C64 = 1 /* ComplexF32 */, GLA_C128 = 2 /* ComplexF64 */ } gla_dtype;
+
+/* A cuboid volume, GlaVol (src/glaMem.jl:12-19, ctor :136-147): cell counts,
+ * cell size and centre as exact rationals (numerator / denominator), in
+ * wavelengths. */
+typedef struct {
+    int64_t cel[3];
+    int64_t scl_num[3], scl_den[3];
+    int64_t org_num[3], org_den[3];
+} gla_vol;
+
+/* Operator options, GlaKerOpt (src/glaMem.jl:62-70).  frq_phz and int_ord
+ * only affect Green-tensor generation; adj_mod is carried as a flag exactly
+ * as the reference carries it (the matvec never reads it,
+ * src/glaLinAlg.jl:16-29); device selects the HIP device. */
+typedef struct {
+    double frq_phz_re, frq_phz_im;
+    int32_t int_ord;
+    int32_t adj_mod;
+    int32_t dtype;      /* gla_dtype: the operator's setTyp */
+    int32_t device;     /* HIP device ordinal */
+} gla_opts;
+
+typedef struct gla_opr_s* gla_opr_t;
+
+/* Geometry of an operator as the matvec sees it (one source and one target
+ * partition): sizes, branch lengths B = (S+T)/2 (src/glaAct.jl:68), the
+ * stored Green extents U (src/glaMemSup.jl:121-134) and the per-axis
+ * symmetric-storage flags. */
+typedef struct {
+    int64_t src_cel[3], trg_cel[3];
+    int64_t brn[3];
+    int64_t fur_ext[3];
+    int32_t sym[3];
+    int32_t dtype;
+    int32_t adj_mod;
+    int32_t self_opr;
+    int64_t dev_brn[3];   /* branch lengths actually transformed on the GPU */
+} gla_dims;
+
+/* ---------------------------------------------------------------------- */
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* gla_last_error(void);
+
+/* Library / device check.  Returns GLA_ERR_NO_DEVICE when no HIP device is
+ * usable (the reference's "CUDA not functional", src/glaMemSup.jl:22-24). */
+gla_status gla_init(int32_t device);
+
+/* Geometry that gla_opr_create_from_fur expects for (trg, src): the reference
+ * derives it in GlaOprMem (src/glaMemSup.jl:16-156).  src == NULL means a
+ * self operator.  Fails with GLA_ERR_SCALE for cell scales that are not
+ * equal (the multi-partition form, src/glaMem.jl:190-216, is not
+ * implemented) and GLA_ERR_SIZE for an odd S+T on an axis. */
+gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* out);
+
+/* Operator from precomputed Fourier Green data: the egoFur= re-entry of
+ * GlaOprMem (src/glaMemSup.jl:16-20, 54, 158-164).  fur holds the 8 branches
+ * back to back, each (U1,U2,U3,6) column-major with components
+ * [xx,yy,zz,xy,xz,yz]; fur_dtype gives its element type; fur_len is the
+ * number of complex elements (8*U1*U2*U3*6).  Non-finite values fail with
+ * GLA_ERR_NONFINITE as the reference's check does. */
+gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, const gla_vol* src,
+                                   const void* fur, int32_t fur_dtype, int64_t fur_len,
+                                   gla_opr_t* out);
+
+/* Operator from geometry: the full GlaOprMem constructor
+ * (src/glaMemSup.jl:16-171) -- FP64 Green-tensor generation on the GPU
+ * (genEgoSlf!/genEgoExt!, src/glaGen.jl:17-48), circulant embedding, FFT,
+ * branch extraction and truncation, then rounding to opts->dtype. */
+gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vol* src, gla_opr_t* out);
+
+/* Copy out the Fourier Green data in the reference's layout (for caching, the
+ * preload-file equivalent of examples/GilaOperators.jl:28-55). buf receives
+ * 8*U1*U2*U3*6 elements of the operator's dtype; len is its capacity. */
+gla_status gla_opr_export_fur(gla_opr_t h, void* buf, int64_t len);
+
+gla_status gla_opr_get_dims(gla_opr_t h, gla_dims* out);
+
+/* The adjoint wrapper of the reference (src/glaLinAlg.jl:16-29): a new
+ * handle that shares the Green data, with adj_mod flipped and target /
+ * source volumes swapped.  The matvec is unchanged, as in the reference. */
+gla_status gla_opr_adjoint(gla_opr_t h, gla_opr_t* out);
+
+gla_status gla_opr_destroy(gla_opr_t h);
+
+/* Matvec, egoOpr! (src/glaAct.jl:33-49) on device memory.
+ * x: (S1,S2,S3,3,nrhs), y: (T1,T2,T3,3,nrhs), both device pointers of the
+ * operator's dtype, caller-owned; x is not modified.  nx / ny give the
+ * number of complex elements of each buffer (a mismatch is GLA_ERR_SIZE, the
+ * reference's size check).  The call is asynchronous on `stream` (a
+ * hipStream_t, NULL for the default stream); the reference blocks, so its
+ * binding synchronizes the stream before returning.  Work buffers belong to
+ * the handle: calls that overlap on one handle must be serialized by the
+ * caller (use separate handles for concurrent streams). */
+gla_status gla_opr_apply(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
+                         void* stream);
+
+/* mul!(y, G, x, alpha, beta) (src/glaLinAlg.jl:61-69): y = alpha*G*x + beta*y,
+ * fused into the last pass.  beta == 0 never reads y. */
+gla_status gla_opr_apply_axpby(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
+                               double alpha_re, double alpha_im, double beta_re, double beta_im,
+                               void* stream);
+
+/* G * x on host memory (src/glaLinAlg.jl:33-43 with the host-to-device copy
+ * of :37-40): copies in, applies, copies out and synchronizes. */
+gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs);
+
+/* Per-kernel timing of the matvec with HIP events on `stream`: after one
+ * full matvec, each of the 5 passes (0 x-forward, 1 y-forward, 2 z fused
+ * forward/multiply/inverse, 3 y-inverse, 4 x-inverse) is launched `reps`
+ * times back to back between two events; ms[p] receives its average launch
+ * duration and bytes[p] its algorithmic HBM bytes (compulsory reads and
+ * writes of that pass; pass 2 includes the stored Green data).  y holds a
+ * valid matvec on return.  No counterpart in the reference (which has no
+ * profiling); for benchmarks. */
+gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, void* stream,
+                           int32_t reps, double* ms, int64_t* bytes);
+
+/* Bytes of device work memory the handle holds for nrhs right-hand sides. */
+int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GILA_AMD_H */

@@ -1,0 +1,189 @@
+"""GPU parity of the HIP matvec against the CPU oracle, through the C ABI.
+
+Tolerances (relative L2 against the FP64 oracle fed the same Fourier Green
+data): ComplexF64 <= 1e-12, ComplexF32 <= 1e-5 (x and G^ rounded to FP32)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+H = 1.0 / 32
+TOL = {np.complex128: 1e-12, np.complex64: 1e-5}
+
+
+def _dev_apply(gila, mem, x, nrhs=1):
+    import torch
+    from gila_amd import _capi
+    typ = mem.setTyp
+    xd = torch.from_numpy(np.asfortranarray(x.astype(typ)).ravel(order="F").copy()).cuda()
+    T = tuple(mem._dims.trg_cel)
+    ny = int(np.prod(T)) * 3 * nrhs
+    yd = torch.empty(ny, dtype=xd.dtype, device=xd.device)
+    _capi.check(gila.lib().gla_opr_apply(mem._h, ctypes.c_void_p(xd.data_ptr()), xd.numel(),
+                                         ctypes.c_void_p(yd.data_ptr()), ny, nrhs, None))
+    torch.cuda.synchronize()
+    shape = T + (3,) + ((nrhs,) if nrhs > 1 else ())
+    return yd.cpu().numpy().reshape(shape, order="F")
+
+
+def _mem_self(gila, orc, N, typ, model=0):
+    fur, crc = orc.self_operator_fur(N, model=model)
+    U = orc.dims(N, N, (1, 1, 1))[1]
+    vol = gila.GlaVol(N, ((1, 32),) * 3)
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), vol, egoFur=orc.fur_branches(fur.astype(typ), U), setTyp=typ)
+    return mem, fur
+
+
+SELF_SHAPES = [(1, 1, 1), (2, 2, 2), (4, 4, 4), (3, 5, 6), (16, 16, 16), (8, 12, 16), (10, 20, 40), (16, 32, 16),
+               (24, 48, 64), (80, 40, 40), (96, 8, 6)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", SELF_SHAPES)
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_self_matvec_parity(gila, orc, gpu, N, typ):
+    print(f"case self N={N} {np.dtype(typ).name}", flush=True)
+    mem, fur = _mem_self(gila, orc, N, typ)
+    x = orc.complex_gaussian(N + (3,), seed=1)
+    y_ref = orc.matvec(N, N, (1, 1, 1), fur, x)
+    y = _dev_apply(gila, mem, x)
+    assert orc.rel_l2(y, y_ref) <= TOL[typ]
+
+
+# external pairs exercising zero-padding, spill-over and bottom-half crops on
+# every axis (B = (S+T)/2 must be a compiled length)
+EXT = [((2, 6, 4), (6, 2, 12)), ((6, 4, 12), (2, 12, 4)), ((16, 16, 16), (16, 16, 16)), ((1, 3, 5), (3, 1, 3)),
+       ((24, 10, 30), (40, 30, 10))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("S,T", EXT)
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_external_matvec_parity(gila, orc, gpu, S, T, typ):
+    print(f"case ext S={S} T={T} {np.dtype(typ).name}", flush=True)
+    delta = (0.25, -0.5, 0.75)
+    crc = orc.crc_ext(S, T, (H, H, H), delta)
+    sym = (0, 0, 0)
+    fur = orc.fur_from_crc(S, T, sym, crc)
+    B, U = orc.dims(S, T, sym)
+    src = gila.GlaVol(S, ((1, 32),) * 3, (0, 0, 0))
+    # a target origin that makes no axis symmetric
+    trg = gila.GlaVol(T, ((1, 32),) * 3, ((1, 4), (1, 2), (3, 4)))
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), trg, src, egoFur=orc.fur_branches(fur.astype(typ), U), setTyp=typ)
+    assert tuple(mem._dims.sym) == (0, 0, 0)
+    x = orc.complex_gaussian(tuple(S) + (3,), seed=2)
+    y_ref = orc.matvec(S, T, sym, fur, x)
+    y = _dev_apply(gila, mem, x)
+    assert orc.rel_l2(y, y_ref) <= TOL[typ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_external_touching_symmetric_axes(gila, orc, gpu, typ):
+    # volB <- volA of test/runtests.jl (touching along y; x and z symmetric)
+    A = (16, 16, 16)
+    crc = orc.crc_ext(A, A, (H, H, H), (0.0, 16 * H, 0.0))
+    sym = (1, 0, 1)
+    fur = orc.fur_from_crc(A, A, sym, crc)
+    U = orc.dims(A, A, sym)[1]
+    va = gila.GlaVol(A, ((1, 32),) * 3, (0, 0, 0))
+    vb = gila.GlaVol(A, ((1, 32),) * 3, (0, (16, 32), 0))
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), vb, va, egoFur=orc.fur_branches(fur.astype(typ), U), setTyp=typ)
+    assert tuple(mem._dims.sym) == sym
+    x = orc.complex_gaussian(A + (3,), seed=3)
+    assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(A, A, sym, fur, x)) <= TOL[typ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_batched_rhs(gila, orc, gpu, typ):
+    N = (12, 8, 16)
+    mem, fur = _mem_self(gila, orc, N, typ)
+    nrhs = 5
+    x = orc.complex_gaussian(N + (3, nrhs), seed=4)
+    y = _dev_apply(gila, mem, x, nrhs=nrhs)
+    for r in range(nrhs):
+        y_ref = orc.matvec(N, N, (1, 1, 1), fur, x[..., r])
+        assert orc.rel_l2(y[..., r], y_ref) <= TOL[typ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_full_size_128_cubed(gila, orc, gpu, typ):
+    # the 128^3 self operator at full size against the FP64 oracle
+    N = (128, 128, 128)
+    print(f"case self 128^3 {np.dtype(typ).name}", flush=True)
+    mem, fur = _mem_self(gila, orc, N, typ)
+    x = orc.complex_gaussian(N + (3,), seed=2)
+    y_ref = orc.matvec(N, N, (1, 1, 1), fur, x)
+    y = _dev_apply(gila, mem, x)
+    assert orc.rel_l2(y, y_ref) <= TOL[typ]
+
+
+@pytest.mark.gpu
+def test_operator_surface(gila, orc, gpu):
+    """test/oprTest.jl generic_tests on a device operator."""
+    import torch
+    N = (16, 16, 16)
+    typ = np.complex64
+    mem, fur = _mem_self(gila, orc, N, typ)
+    opr = gila.GlaOpr(mem)
+    assert gila.size(opr) == (16 * 16 * 16 * 3,) * 2
+    assert gila.glaSze(opr) == ((16, 16, 16, 3), (16, 16, 16, 3))
+    assert gila.isselfoperator(opr) and not gila.isexternaloperator(opr)
+    x = torch.ones(gila.size(opr, 2), dtype=torch.complex64, device="cuda")
+    xg = x.reshape(16, 16, 16, 3)     # all-ones: layout is immaterial
+    ref = gila.egoOpr_(mem, xg.clone())
+    acted = opr * xg
+    assert torch.allclose(acted, ref)
+    assert torch.equal(x, torch.ones_like(x))          # input not mutated
+    acted_flat = opr * x
+    assert acted_flat.shape == x.shape
+    with pytest.raises(AssertionError):
+        opr * torch.ones(gila.size(opr, 2), dtype=torch.complex128, device="cuda")
+    with pytest.raises(gila.DimensionMismatch):
+        opr * torch.ones(gila.size(opr, 2) + 1, dtype=torch.complex64, device="cuda")
+    adj = gila.adjoint(opr)
+    adj2 = gila.adjoint(adj)
+    assert gila.isadjoint(adj) and not gila.isadjoint(opr) and not gila.isadjoint(adj2)
+    assert all(np.array_equal(a, b) for a, b in zip(adj2.mem.egoFur, opr.mem.egoFur))
+    assert torch.allclose(adj2 * x, opr * x)
+    assert not gila.issymmetric(opr) and not gila.isposdef(opr) and not gila.ishermitian(opr)
+    assert not gila.isdiag(opr)
+    assert "Self GlaOpr" in repr(opr)
+
+
+@pytest.mark.gpu
+def test_mul_axpby_fused(gila, orc, gpu):
+    import torch
+    N = (8, 8, 8)
+    typ = np.complex128
+    mem, fur = _mem_self(gila, orc, N, typ)
+    opr = gila.GlaOpr(mem)
+    x = orc.complex_gaussian(N + (3,), seed=5)
+    y0 = orc.complex_gaussian(N + (3,), seed=6)
+    a, b = 0.5 - 2j, 1.5 + 0.25j
+    ref = a * orc.matvec(N, N, (1, 1, 1), fur, x) + b * y0
+    xd = torch.from_numpy(x.ravel(order="F").copy()).cuda()
+    yd = torch.from_numpy(y0.ravel(order="F").copy()).cuda()
+    gila.mul_(yd, opr, xd, a, b)
+    torch.cuda.synchronize()
+    assert orc.rel_l2(yd.cpu().numpy().reshape(N + (3,), order="F"), ref) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_host_arrays_and_errors(gila, orc, gpu):
+    N = (6, 4, 8)
+    typ = np.complex128
+    mem, fur = _mem_self(gila, orc, N, typ)
+    x = orc.complex_gaussian(N + (3,), seed=7)
+    y = gila.egoOpr_(mem, x)          # numpy in, numpy out
+    assert orc.rel_l2(y, orc.matvec(N, N, (1, 1, 1), fur, x)) <= 1e-12
+    with pytest.raises(gila.GlaError) as ei:
+        gila.egoOpr_(mem, np.zeros((6, 4, 7, 3), typ))
+    assert ei.value.status == 1
+    bad = [f.copy() for f in mem.egoFur]
+    bad[3][0, 0, 0, 0] = np.nan
+    with pytest.raises(gila.GlaError) as ei:
+        gila.GlaOprMem(gila.GlaKerOpt(True), mem.trgVol, egoFur=bad, setTyp=typ)
+    assert ei.value.status == 2

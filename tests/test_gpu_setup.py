@@ -1,0 +1,82 @@
+"""GPU checks of operator construction from geometry (FP64 Green-tensor setup,
+circulant embedding, transform and branch extraction) against the oracle's
+face-pair model with the same quadrature rule."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+H = Fraction(1, 32)
+
+
+def _fur_flat(mem):
+    return np.concatenate([f.reshape(-1, order="F") for f in mem.egoFur])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", [(8, 8, 8), (16, 16, 16), (6, 10, 4)])
+def test_self_setup_matches_oracle(gila, orc, gpu, N):
+    print(f"case setup self N={N}", flush=True)
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), gila.GlaVol(N, (H,) * 3), setTyp=np.complex128)
+    fur = _fur_flat(mem)
+    toe = orc.toe_model(N, (float(H),) * 3, model=2, qn=10)
+    ref = orc.fur_from_crc(N, N, (1, 1, 1), orc.crc_self(N, toe))
+    assert np.abs(fur - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_external_touching_setup_matches_oracle(gila, orc, gpu):
+    # volB <- volA of test/runtests.jl:10-22 at 1/32 cells, touching along y
+    A = (8, 8, 8)
+    va = gila.GlaVol(A, (H,) * 3, (0, 0, 0))
+    vb = gila.GlaVol(A, (H,) * 3, (0, 8 * H, 0))
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), vb, va, setTyp=np.complex128)
+    fur = _fur_flat(mem)
+    crc = orc.crc_ext(A, A, (float(H),) * 3, (0.0, 8 * float(H), 0.0), model=2, qn=10)
+    ref = orc.fur_from_crc(A, A, (1, 0, 1), crc)
+    assert np.abs(fur - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_self_vs_external_identity_on_gpu(gila, orc, gpu):
+    # test/extSlfTest.jl:45-47 with operators built by the library
+    A, U = (8, 8, 8), (8, 16, 8)
+    va = gila.GlaVol(A, (H,) * 3, (0, 0, 0))
+    vb = gila.GlaVol(A, (H,) * 3, (0, 8 * H, 0))
+    vu = gila.GlaVol(U, (H,) * 3, (0, 4 * H, 0))
+    ext = gila.GlaOpr(gila.GlaOprMem(gila.GlaKerOpt(False), vb, va, setTyp=np.complex128))
+    uni = gila.GlaOpr(gila.GlaOprMem(gila.GlaKerOpt(False), vu, setTyp=np.complex128))
+    xa = orc.complex_gaussian(A + (3,), seed=8)
+    xu = np.zeros(U + (3,), np.complex128, order="F")
+    xu[:, :8] = xa
+    yu = uni * xu
+    ye = ext * xa
+    assert orc.rel_l2(ye, yu[:, 8:]) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_overlap_is_rejected_unless_grids_align(gila, gpu):
+    a = gila.GlaVol((8, 8, 8), (H,) * 3, (0, 0, 0))
+    b = gila.GlaVol((8, 8, 8), (H,) * 3, (Fraction(1, 64), 0, 0))   # half-cell shift, overlapping
+    with pytest.raises(gila.GlaError) as ei:
+        gila.GlaOprMem(gila.GlaKerOpt(True), b, a, setTyp=np.complex64)
+    assert ei.value.status == 4
+    # a source inside a larger target on the same grid is the restricted self operator
+    c = gila.GlaVol((4, 4, 4), (H,) * 3, (0, 0, 0))
+    t = gila.GlaVol((8, 8, 8), (H,) * 3, (0, 0, 0))
+    m = gila.GlaOprMem(gila.GlaKerOpt(True), t, c, setTyp=np.complex128)
+    assert tuple(m._dims.trg_cel) == (8, 8, 8)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_setup_operator_matvec(gila, orc, gpu, typ):
+    N = (16, 16, 16)
+    opr = gila.GlaOpr(N, (H,) * 3, useGpu=True, setTyp=typ)
+    fur64 = _fur_flat(gila.GlaOpr(N, (H,) * 3, setTyp=np.complex128).mem)
+    x = orc.complex_gaussian(N + (3,), seed=9)
+    import torch
+    xd = torch.from_numpy(x.astype(typ).ravel(order="F").copy()).cuda()
+    y = (opr * xd).cpu().numpy().reshape(N + (3,), order="F")
+    y_ref = orc.matvec(N, N, (1, 1, 1), fur64, x)
+    assert orc.rel_l2(y, y_ref) <= (1e-12 if typ == np.complex128 else 1e-5)
