@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,12 +106,15 @@ def main():
 
     res = {}
     for typ in (np.complex64, np.complex128):
+        log(f"setup {np.dtype(typ).name} {N}")
         t0 = time.perf_counter()
         mem = make_operator(gila, N, typ)
         setup_s = time.perf_counter() - t0
+        log(f"setup done in {setup_s:.2f} s; timing {a.steps} steps")
         xd = torch.from_numpy(np.asfortranarray(x.astype(typ)).ravel(order="F").copy()).cuda()
         yd = torch.empty_like(xd)
         ms, wall = time_matvec(gila, mem, xd, yd, a.steps, a.warmup)
+        log(f"timed {ms:.4f} ms/step; profiling passes")
         prof = profile(gila, mem, xd, yd)
         y = yd.cpu().numpy().reshape(N + (3,), order="F")
         res[typ] = dict(mem=mem, ms=ms, wall=wall, prof=prof, y=y, setup_s=setup_s)
@@ -150,6 +157,7 @@ def main():
         out["roofline"]["traffic"] = tr.get(dom["kernel"])
 
     if not a.no_cpu:
+        log("cpu leg: FP64 oracle matvec")
         orc = ge.load_oracle()     # the checker: CPU baseline and parity only
         fur64 = np.concatenate([f.reshape(-1, order="F") for f in r64["mem"].egoFur])
         sym = (1, 1, 1)
@@ -158,6 +166,7 @@ def main():
         x32 = x.astype(np.complex64)
         orc.matvec(N, N, sym, fur32, x32, dtype=np.complex64)   # warm-up
         ts = []
+        log("cpu leg: timing the FP32 restatement")
         for _ in range(a.cpu_reps):
             t0 = time.perf_counter()
             orc.matvec(N, N, sym, fur32, x32, dtype=np.complex64)

@@ -444,12 +444,32 @@ gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, i
     return GLA_OK;
 }
 
+}  // extern "C"
+namespace {
+gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, void* stream, int32_t reps, double* ms,
+                          int64_t* bytes);
+}
+extern "C" {
+
 gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, void* stream,
                            int32_t reps, double* ms, int64_t* bytes)
 {
     if (!h || !ms || !bytes || reps < 1) return fail(GLA_ERR_ARG, "bad profile arguments");
     gla_status s = run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);   // validates, allocates, warms up
     if (s) return s;
+    s = profile_locked(h, x, y, nrhs, stream, reps, ms, bytes);
+    if (s) return s;
+    // leave y holding a complete matvec again (pass 2 ran in place reps times)
+    return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, void* stream, int32_t reps, double* ms,
+                          int64_t* bytes)
+{
     std::lock_guard<std::mutex> lk(h->mu);
     DevGuard g(h->opts.device);
     const gla::Geom& G = h->geom;
@@ -488,9 +508,12 @@ gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int6
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // leave y holding a complete matvec again (pass 2 ran in place reps times)
-    return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);
+    return GLA_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs)
 {

@@ -187,3 +187,24 @@ def test_host_arrays_and_errors(gila, orc, gpu):
     with pytest.raises(gila.GlaError) as ei:
         gila.GlaOprMem(gila.GlaKerOpt(True), mem.trgVol, egoFur=bad, setTyp=typ)
     assert ei.value.status == 2
+
+
+@pytest.mark.gpu
+def test_profile_entry_leaves_a_valid_matvec(gila, orc, gpu):
+    import torch
+    from gila_amd import _capi
+    N = (16, 16, 16)
+    mem, fur = _mem_self(gila, orc, N, np.complex64)
+    x = orc.complex_gaussian(N + (3,), seed=10)
+    xd = torch.from_numpy(x.astype(np.complex64).ravel(order="F").copy()).cuda()
+    yd = torch.empty_like(xd)
+    ms = (ctypes.c_double * 5)()
+    by = (ctypes.c_int64 * 5)()
+    st = torch.cuda.current_stream().cuda_stream
+    _capi.check(gila.lib().gla_opr_profile(mem._h, ctypes.c_void_p(xd.data_ptr()), xd.numel(),
+                                           ctypes.c_void_p(yd.data_ptr()), yd.numel(), 1, ctypes.c_void_p(st), 3,
+                                           ms, by))
+    torch.cuda.synchronize()
+    assert all(ms[i] > 0 for i in range(5)) and all(by[i] > 0 for i in range(5))
+    y = yd.cpu().numpy().reshape(N + (3,), order="F")
+    assert orc.rel_l2(y, orc.matvec(N, N, (1, 1, 1), fur, x)) <= 1e-5
