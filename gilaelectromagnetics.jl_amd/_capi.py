@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgila_amd.so")
+# GILA_AMD_LIB selects another build of the library (kernel-variant timing)
+LIB_PATH = os.environ.get("GILA_AMD_LIB") or os.path.join(_HERE, "libgila_amd.so")
 
 GLA_OK = 0
 GLA_ERR_SIZE = 1

@@ -239,21 +239,44 @@ __device__ __forceinline__ void dft(cpx<R>* a, const cpx<R>* __restrict__ tw)
 }
 
 // ---------------------------------------------------------------------------
-// The pencil transform.
-//   v    : E = N/TP register elements, v[e] = x[t + TP*e]
-//   lds  : WG scratch; element pos of this pencil lives at lds[pos*NS + pen*PS]
-//   tw   : table tw[n] = e^{-2 pi i n / N}, n < N (forward sign)
-// All threads of the workgroup must call with the same template arguments
-// (the barriers are workgroup wide).  `act` false masks a pencil beyond the
-// end of the data: its threads still take part in every barrier.
+// LDS exchange fences.  SYNC 0: workgroup barrier (a pencil's threads span
+// several waves).  SYNC 1: the pencil's threads are lanes of one wave, whose
+// LDS operations complete in issue order, so a compiler fence ordering the
+// writes before the reads is enough.
 // ---------------------------------------------------------------------------
-template <typename R, int N, int TP, int NS, int PS, bool INV>
+template <int SYNC>
+__device__ __forceinline__ void lds_sync()
+{
+    if constexpr (SYNC == 0) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The pencil transform.
+//   v    : K independent pencils per thread, E = N/TP register elements each,
+//          v[k][e] = x_k[t + TP*e]
+//   lds  : WG scratch; element pos of pencil k lives at
+//          lds[k*KS + pos*NS + pen*PS]
+//   tw   : table tw[n] = e^{-2 pi i n / N}, n < N (forward sign)
+// The K pencils go through each stage together: one twiddle load serves all
+// of them and one pair of fences per stage covers all of them.  All threads
+// of the workgroup (SYNC 0) or of the wave (SYNC 1) must call with the same
+// template arguments.  A masked pencil beyond the data still takes part.
+// ---------------------------------------------------------------------------
+template <typename R, int N, int TP, int NS, int PS, bool INV, int SYNC = 0, int K = 1, int KS = 0>
 struct PencilFft {
     static constexpr int E = N / TP;
     static_assert(N % TP == 0, "TP must divide N");
     static constexpr Plan P = make_plan(N, E);
     static constexpr bool FIRST_REG = (P.nst > 0) && (E % P.rad[0] == 0);
     static constexpr bool LAST_REG = (P.nst > 1) && (E % P.rad[P.nst - 1] == 0);
+
+    static __device__ __forceinline__ cpx<R> twid(cpx<R> a, cpx<R> w) { return INV ? cmulc(a, w) : a * w; }
 
     // one stage reading LDS and writing LDS (any thread mapping)
     template <int S>
@@ -263,35 +286,41 @@ struct PencilFft {
         constexpr int NB = N / RAD;                      // butterflies
         constexpr int NU = (NB + TP - 1) / TP;          // per thread (upper bound)
         constexpr int NSP = [] { int ns = 1; for (int i = 0; i < S; ++i) ns *= P.rad[i]; return ns; }();
-        cpx<R> a[NU][RAD];
+        cpx<R> a[K][NU][RAD];
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
             const int j = t + TP * u;
             if (NB % TP == 0 || j < NB) {
 #pragma unroll
-                for (int q = 0; q < RAD; ++q) a[u][q] = lds[(j + q * NB) * NS + pen * PS];
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int q = 0; q < RAD; ++q) a[k][u][q] = lds[k * KS + (j + q * NB) * NS + pen * PS];
             }
         }
-        __syncthreads();
+        lds_sync<SYNC>();
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
             const int j = t + TP * u;
             if (NB % TP == 0 || j < NB) {
-                const int k = j % NSP;
+                const int kk = j % NSP;
                 if constexpr (NSP > 1) {
 #pragma unroll
                     for (int q = 1; q < RAD; ++q) {
-                        cpx<R> w = tw[(q * k * (N / (NSP * RAD))) % N];
-                        a[u][q] = INV ? cmulc(a[u][q], w) : a[u][q] * w;
+                        const cpx<R> w = tw[q * kk * (N / (NSP * RAD))];
+#pragma unroll
+                        for (int k = 0; k < K; ++k) a[k][u][q] = twid(a[k][u][q], w);
                     }
                 }
-                dft<R, INV, RAD, N>(a[u], tw);
-                const int base = (j / NSP) * NSP * RAD + k;
+                const int base = (j / NSP) * NSP * RAD + kk;
 #pragma unroll
-                for (int q = 0; q < RAD; ++q) lds[(base + q * NSP) * NS + pen * PS] = a[u][q];
+                for (int k = 0; k < K; ++k) {
+                    dft<R, INV, RAD, N>(a[k][u], tw);
+#pragma unroll
+                    for (int q = 0; q < RAD; ++q) lds[k * KS + (base + q * NSP) * NS + pen * PS] = a[k][u][q];
+                }
             }
         }
-        __syncthreads();
+        lds_sync<SYNC>();
     }
 
     template <int S>
@@ -304,35 +333,40 @@ struct PencilFft {
         }
     }
 
-    static __device__ __forceinline__ void run(cpx<R> (&v)[E], cpx<R>* lds, int t, int pen,
+    static __device__ __forceinline__ void run(cpx<R> (&v)[K][E], cpx<R>* lds, int t, int pen,
                                                const cpx<R>* __restrict__ tw)
     {
         if constexpr (P.nst == 0) {
             return;
         } else if constexpr (P.nst == 1 && FIRST_REG) {
             // whole transform inside one thread (E == N)
-            dft<R, INV, P.rad[0], N>(v, tw);
+#pragma unroll
+            for (int k = 0; k < K; ++k) dft<R, INV, P.rad[0], N>(v[k], tw);
         } else {
             // ---- first stage
             if constexpr (FIRST_REG) {
                 constexpr int RAD = P.rad[0];
                 constexpr int EU = E / RAD;   // butterflies per thread
 #pragma unroll
-                for (int u = 0; u < EU; ++u) {
-                    cpx<R> a[RAD];
+                for (int k = 0; k < K; ++k)
 #pragma unroll
-                    for (int q = 0; q < RAD; ++q) a[q] = v[u + q * EU];
-                    dft<R, INV, RAD, N>(a, tw);
-                    const int j = t + TP * u;    // butterfly index, span 1
+                    for (int u = 0; u < EU; ++u) {
+                        cpx<R> a[RAD];
 #pragma unroll
-                    for (int q = 0; q < RAD; ++q) lds[(j * RAD + q) * NS + pen * PS] = a[q];
-                }
-                __syncthreads();
+                        for (int q = 0; q < RAD; ++q) a[q] = v[k][u + q * EU];
+                        dft<R, INV, RAD, N>(a, tw);
+                        const int j = t + TP * u;    // butterfly index, span 1
+#pragma unroll
+                        for (int q = 0; q < RAD; ++q) lds[k * KS + (j * RAD + q) * NS + pen * PS] = a[q];
+                    }
+                lds_sync<SYNC>();
                 mid_stages<1>(lds, t, pen, tw);
             } else {
 #pragma unroll
-                for (int e = 0; e < E; ++e) lds[(t + TP * e) * NS + pen * PS] = v[e];
-                __syncthreads();
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int e = 0; e < E; ++e) lds[k * KS + (t + TP * e) * NS + pen * PS] = v[k][e];
+                lds_sync<SYNC>();
                 mid_stages<0>(lds, t, pen, tw);
             }
             // ---- last stage
@@ -344,24 +378,40 @@ struct PencilFft {
 #pragma unroll
                 for (int u = 0; u < EU; ++u) {
                     const int j = t + TP * u;
-                    cpx<R> a[RAD];
+                    cpx<R> a[K][RAD];
 #pragma unroll
-                    for (int q = 0; q < RAD; ++q) a[q] = lds[(j + q * NSP) * NS + pen * PS];
+                    for (int k = 0; k < K; ++k)
+#pragma unroll
+                        for (int q = 0; q < RAD; ++q) a[k][q] = lds[k * KS + (j + q * NSP) * NS + pen * PS];
 #pragma unroll
                     for (int q = 1; q < RAD; ++q) {
-                        cpx<R> w = tw[(q * j) % N];
-                        a[q] = INV ? cmulc(a[q], w) : a[q] * w;
-                    }
-                    dft<R, INV, RAD, N>(a, tw);
+                        const cpx<R> w = tw[q * j];
 #pragma unroll
-                    for (int q = 0; q < RAD; ++q) v[u + q * EU] = a[q];
+                        for (int k = 0; k < K; ++k) a[k][q] = twid(a[k][q], w);
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        dft<R, INV, RAD, N>(a[k], tw);
+#pragma unroll
+                        for (int q = 0; q < RAD; ++q) v[k][u + q * EU] = a[k][q];
+                    }
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < E; ++e) v[e] = lds[(t + TP * e) * NS + pen * PS];
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int e = 0; e < E; ++e) v[k][e] = lds[k * KS + (t + TP * e) * NS + pen * PS];
             }
-            __syncthreads();   // LDS free for the next transform
+            lds_sync<SYNC>();   // LDS free for the next transform
         }
+    }
+
+    // single pencil
+    static __device__ __forceinline__ void run(cpx<R> (&v)[E], cpx<R>* lds, int t, int pen,
+                                               const cpx<R>* __restrict__ tw)
+    {
+        static_assert(K == 1, "single-pencil form needs K == 1");
+        run(reinterpret_cast<cpx<R>(&)[1][E]>(v), lds, t, pen, tw);
     }
 };
 

@@ -44,16 +44,18 @@ __device__ __forceinline__ void axis_lookup(const AxisMap& a, int p, int k, int&
 // --------------------------------------------------------------------------
 // Pass 1: x-forward on contiguous rows.  One pencil = one (i2,i3,c) row of X.
 // --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PEN>
-__global__ __launch_bounds__(TP * PEN) void k_fwd_x(const cpx<R>* __restrict__ X, cpx<R>* __restrict__ W1,
-                                                    int S1, int S2, int S3, int Y1, int Z,
-                                                    int64_t xs, int64_t ws,
-                                                    const cpx<R>* __restrict__ phz,
-                                                    const cpx<R>* __restrict__ tw)
+template <typename R, int N, int TP, int PEN, int MINW>
+__global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restrict__ X, cpx<R>* __restrict__ W1,
+                                                          int S1, int S2, int S3, int Y1, int Z,
+                                                          int64_t xs, int64_t ws,
+                                                          const cpx<R>* __restrict__ phz,
+                                                          const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
     constexpr int PS = N + 1;
-    __shared__ cpx<R> lds[PEN * PS];
+    // rows are whole pencils inside one wave when TP divides 64
+    constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
+    __shared__ cpx<R> lds[2 * PEN * PS];
     const int t = threadIdx.x % TP, pen = threadIdx.x / TP;
     const int rows = S2 * S3 * 3;
     const int row = blockIdx.x * PEN + pen;
@@ -61,42 +63,41 @@ __global__ __launch_bounds__(TP * PEN) void k_fwd_x(const cpx<R>* __restrict__ X
     const int i2 = act ? row % S2 : 0, i3 = act ? (row / S2) % S3 : 0, c = act ? row / (S2 * S3) : 0;
     const cpx<R>* xr = X + (int64_t)blockIdx.y * xs + (int64_t)row * S1;
     cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
-    cpx<R> ve[E], vo[E];
+    cpx<R> v[2][E];   // even, odd parity
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
         cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
         if (act && n < S1) a = xr[n];
         if (act && n + N < S1) b = xr[n + N];
-        ve[e] = a + b;
-        vo[e] = phz[n] * (a - b);
+        v[0][e] = a + b;
+        v[1][e] = phz[n] * (a - b);
     }
-    PencilFft<R, N, TP, 1, PS, false>::run(ve, lds, t, pen, tw);
+    PencilFft<R, N, TP, 1, PS, false, XSYNC, 2, PEN * PS>::run(v, lds, t, pen, tw);
     if (act) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) wr[t + TP * e] = ve[e];
-    }
-    PencilFft<R, N, TP, 1, PS, false>::run(vo, lds, t, pen, tw);
-    if (act) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) wr[N + t + TP * e] = vo[e];
+        for (int e = 0; e < E; ++e) {
+            wr[t + TP * e] = v[0][e];
+            wr[N + t + TP * e] = v[1][e];
+        }
     }
 }
 
 // --------------------------------------------------------------------------
 // Pass 5: x-inverse + merge/crop onto T1, optional y = alpha*res + beta*y.
 // --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PEN>
-__global__ __launch_bounds__(TP * PEN) void k_inv_x(const cpx<R>* __restrict__ W1, cpx<R>* __restrict__ Y,
-                                                    int T1, int T2, int T3, int Y1, int Z,
-                                                    int64_t ws, int64_t ys,
-                                                    const cpx<R>* __restrict__ phz,
-                                                    const cpx<R>* __restrict__ tw,
-                                                    cpx<R> alpha, cpx<R> beta, int axpby)
+template <typename R, int N, int TP, int PEN, int MINW>
+__global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restrict__ W1, cpx<R>* __restrict__ Y,
+                                                          int T1, int T2, int T3, int Y1, int Z,
+                                                          int64_t ws, int64_t ys,
+                                                          const cpx<R>* __restrict__ phz,
+                                                          const cpx<R>* __restrict__ tw,
+                                                          cpx<R> alpha, cpx<R> beta, int axpby)
 {
     constexpr int E = N / TP;
     constexpr int PS = N + 1;
-    __shared__ cpx<R> lds[PEN * PS];
+    constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
+    __shared__ cpx<R> lds[2 * PEN * PS];
     const int t = threadIdx.x % TP, pen = threadIdx.x / TP;
     const int rows = T2 * T3 * 3;
     const int row = blockIdx.x * PEN + pen;
@@ -104,29 +105,29 @@ __global__ __launch_bounds__(TP * PEN) void k_inv_x(const cpx<R>* __restrict__ W
     const int i2 = act ? row % T2 : 0, i3 = act ? (row / T2) % T3 : 0, c = act ? row / (T2 * T3) : 0;
     const cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
     cpx<R>* yr = Y + (int64_t)blockIdx.y * ys + (int64_t)row * T1;
-    cpx<R> ve[E], vo[E];
+    cpx<R> v[2][E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int k = t + TP * e;
-        ve[e] = act ? wr[k] : cpx<R>{R(0), R(0)};
-        vo[e] = act ? wr[N + k] : cpx<R>{R(0), R(0)};
+        v[0][e] = act ? wr[k] : cpx<R>{R(0), R(0)};
+        v[1][e] = act ? wr[N + k] : cpx<R>{R(0), R(0)};
     }
-    PencilFft<R, N, TP, 1, PS, true>::run(ve, lds, t, pen, tw);
-    PencilFft<R, N, TP, 1, PS, true>::run(vo, lds, t, pen, tw);
+    PencilFft<R, N, TP, 1, PS, true, XSYNC, 2, PEN * PS>::run(v, lds, t, pen, tw);
     if (!act) return;
     const int top = T1 < N ? T1 : N;
+    const bool rd = axpby && !(beta.x == R(0) && beta.y == R(0));
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
-        const cpx<R> po = cmulc(vo[e], phz[n]);
+        const cpx<R> po = cmulc(v[1][e], phz[n]);
         if (n < top) {
-            cpx<R> r = ve[e] + po;
-            if (axpby) r = alpha * r + (beta.x == R(0) && beta.y == R(0) ? cpx<R>{R(0), R(0)} : beta * yr[n]);
+            cpx<R> r = v[0][e] + po;
+            if (axpby) r = alpha * r + (rd ? beta * yr[n] : cpx<R>{R(0), R(0)});
             yr[n] = r;
         }
         if (n + N < T1) {
-            cpx<R> r = ve[e] - po;
-            if (axpby) r = alpha * r + (beta.x == R(0) && beta.y == R(0) ? cpx<R>{R(0), R(0)} : beta * yr[n + N]);
+            cpx<R> r = v[0][e] - po;
+            if (axpby) r = alpha * r + (rd ? beta * yr[n + N] : cpx<R>{R(0), R(0)});
             yr[n + N] = r;
         }
     }
@@ -135,96 +136,96 @@ __global__ __launch_bounds__(TP * PEN) void k_inv_x(const cpx<R>* __restrict__ W
 // --------------------------------------------------------------------------
 // Pass 2: y-forward on bundles of PX consecutive axis-1 columns.
 // --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PX>
-__global__ __launch_bounds__(TP * PX) void k_fwd_y(const cpx<R>* __restrict__ W1, cpx<R>* __restrict__ W2,
+template <typename R, int N, int TP, int PX, int MINW>
+__global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restrict__ W1, cpx<R>* __restrict__ W2,
                                                    int L1, int S2, int S3, int Y1, int Z, int64_t w1s,
                                                    int64_t w2s, const cpx<R>* __restrict__ phz,
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[PX * N];
+    __shared__ cpx<R> lds[2 * PX * N];
     const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
     const int m1 = blockIdx.x * PX + xl;
     const bool act = m1 < L1;
     const int i3 = blockIdx.y % S3, c = blockIdx.y / S3;
     const cpx<R>* src = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + m1;
     cpx<R>* dst = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + m1;
-    cpx<R> ve[E], vo[E];
+    cpx<R> v[2][E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
         cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
         if (act && n < S2) a = src[(int64_t)n * L1];
         if (act && n + N < S2) b = src[(int64_t)(n + N) * L1];
-        ve[e] = a + b;
-        vo[e] = phz[n] * (a - b);
+        v[0][e] = a + b;
+        v[1][e] = phz[n] * (a - b);
     }
-    PencilFft<R, N, TP, PX, 1, false>::run(ve, lds, t, xl, tw);
+    PencilFft<R, N, TP, PX, 1, false, 0, 2, PX * N>::run(v, lds, t, xl, tw);
     if (act) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[(int64_t)(t + TP * e) * L1] = ve[e];
-    }
-    PencilFft<R, N, TP, PX, 1, false>::run(vo, lds, t, xl, tw);
-    if (act) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) dst[(int64_t)(N + t + TP * e) * L1] = vo[e];
+        for (int e = 0; e < E; ++e) {
+            dst[(int64_t)(t + TP * e) * L1] = v[0][e];
+            dst[(int64_t)(N + t + TP * e) * L1] = v[1][e];
+        }
     }
 }
 
 // --------------------------------------------------------------------------
 // Pass 4: y-inverse + merge/crop onto T2.
 // --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PX>
-__global__ __launch_bounds__(TP * PX) void k_inv_y(const cpx<R>* __restrict__ W2, cpx<R>* __restrict__ W1,
+template <typename R, int N, int TP, int PX, int MINW>
+__global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restrict__ W2, cpx<R>* __restrict__ W1,
                                                    int L1, int T2, int T3, int Y1, int Z, int64_t w2s,
                                                    int64_t w1s, const cpx<R>* __restrict__ phz,
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[PX * N];
+    __shared__ cpx<R> lds[2 * PX * N];
     const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
     const int m1 = blockIdx.x * PX + xl;
     const bool act = m1 < L1;
     const int i3 = blockIdx.y % T3, c = blockIdx.y / T3;
     const cpx<R>* src = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + m1;
     cpx<R>* dst = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + m1;
-    cpx<R> ve[E], vo[E];
+    cpx<R> v[2][E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int k = t + TP * e;
-        ve[e] = act ? src[(int64_t)k * L1] : cpx<R>{R(0), R(0)};
-        vo[e] = act ? src[(int64_t)(N + k) * L1] : cpx<R>{R(0), R(0)};
+        v[0][e] = act ? src[(int64_t)k * L1] : cpx<R>{R(0), R(0)};
+        v[1][e] = act ? src[(int64_t)(N + k) * L1] : cpx<R>{R(0), R(0)};
     }
-    PencilFft<R, N, TP, PX, 1, true>::run(ve, lds, t, xl, tw);
-    PencilFft<R, N, TP, PX, 1, true>::run(vo, lds, t, xl, tw);
+    PencilFft<R, N, TP, PX, 1, true, 0, 2, PX * N>::run(v, lds, t, xl, tw);
     if (!act) return;
     const int top = T2 < N ? T2 : N;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
-        const cpx<R> po = cmulc(vo[e], phz[n]);
-        if (n < top) dst[(int64_t)n * L1] = ve[e] + po;
-        if (n + N < T2) dst[(int64_t)(n + N) * L1] = ve[e] - po;
+        const cpx<R> po = cmulc(v[1][e], phz[n]);
+        if (n < top) dst[(int64_t)n * L1] = v[0][e] + po;
+        if (n + N < T2) dst[(int64_t)(n + N) * L1] = v[0][e] - po;
     }
 }
 
 // --------------------------------------------------------------------------
 // Pass 3: z-forward, G^ multiply, z-inverse, merge/crop onto T3 -- in place
 // on W2 (each workgroup owns its columns: every load precedes the first
-// barrier, every store follows the last).
+// barrier after which any store can happen, every store follows the last
+// load).  Each thread carries all three
+// field components; KB = 3 runs the three transforms through each stage
+// together (three LDS buffers), KB = 1 one after the other (one buffer).
 // MODE 0: S3 == T3 == B3; MODE 1: S3 > B3 (spill-over add); MODE 2: T3 > B3.
 // G^ device layout: gf[(((b*U3 + j3)*U2 + j2)*U1 + j1)*6 + comp],
 // comp order [xx,yy,zz,xy,xz,yz], pre-scaled by 1/(L1 L2 L3).
 // --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PX, int MODE>
-__global__ __launch_bounds__(TP * PX) void k_zpass(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int T3,
+template <typename R, int N, int TP, int PX, int MODE, int MINW, int KB, int RELOAD>
+__global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int T3,
                                                    int Z, int64_t w2s, const cpx<R>* __restrict__ gf,
                                                    AxisMap a1, AxisMap a2, AxisMap a3,
                                                    const cpx<R>* __restrict__ phz,
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[PX * N];
+    __shared__ cpx<R> lds[KB * PX * N];
     const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
     const int m1 = blockIdx.x * PX + xl;
     const int m2 = blockIdx.y;
@@ -233,17 +234,22 @@ __global__ __launch_bounds__(TP * PX) void k_zpass(cpx<R>* __restrict__ W2, int 
     cpx<R>* col = W2 + (int64_t)blockIdx.z * w2s + (int64_t)m2 * L1 + m1;
     const int64_t zs = (int64_t)L2 * L1;
 
-    cpx<R> xa[3][E];
-    cpx<R> xb[MODE == 1 ? 3 : 1][MODE == 1 ? E : 1];
+    // RELOAD: read the column again for the odd parity (an L2 hit) instead of
+    // keeping it in registers through the even parity's transforms
+    constexpr int XE = RELOAD ? 1 : E;
+    cpx<R> xa[3][XE];
+    cpx<R> xb[MODE == 1 ? 3 : 1][MODE == 1 ? XE : 1];
+    if constexpr (!RELOAD) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int n = t + TP * e;
-            xa[c][e] = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
-            if constexpr (MODE == 1)
-                xb[c][e] = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
-        }
+            for (int e = 0; e < E; ++e) {
+                const int n = t + TP * e;
+                xa[c][e] = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
+                if constexpr (MODE == 1)
+                    xb[c][e] = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
+            }
+    }
 
     // per-column branch parities and stored indices on axes 1 and 2
     const int p1 = m1 >= a1.B ? 1 : 0, k1 = m1 - p1 * a1.B;
@@ -258,18 +264,29 @@ __global__ __launch_bounds__(TP * PX) void k_zpass(cpx<R>* __restrict__ W2, int 
 #pragma unroll
     for (int p3 = 0; p3 < 2; ++p3) {
         cpx<R> w[3][E];
+        if constexpr (RELOAD) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int n = t + TP * e;
-                cpx<R> a = xa[c][e];
-                cpx<R> b = {R(0), R(0)};
-                if constexpr (MODE == 1) b = xb[c][e];
+                cpx<R> a, b = {R(0), R(0)};
+                if constexpr (RELOAD) {
+                    a = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
+                    if constexpr (MODE == 1)
+                        b = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
+                } else {
+                    a = xa[c][e];
+                    if constexpr (MODE == 1) b = xb[c][e];
+                }
                 w[c][e] = p3 == 0 ? a + b : phz[n] * (a - b);
             }
+        if constexpr (KB == 3) {
+            PencilFft<R, N, TP, PX, 1, false, 0, 3, PX * N>::run(w, lds, t, xl, tw);
+        } else {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, false>::run(w[c], lds, t, xl, tw);
+            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, false>::run(w[c], lds, t, xl, tw);
+        }
         // symmetric 3x3 multiply (mulKerHst!, src/glaActSup.jl:879-899)
         const int b = (p1 << 2) | (p2 << 1) | p3;
 #pragma unroll
@@ -288,8 +305,12 @@ __global__ __launch_bounds__(TP * PX) void k_zpass(cpx<R>* __restrict__ W2, int 
                 w[2][e] = gxz * x0 + gyz * x1 + gzz * x2;
             }
         }
+        if constexpr (KB == 3) {
+            PencilFft<R, N, TP, PX, 1, true, 0, 3, PX * N>::run(w, lds, t, xl, tw);
+        } else {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, true>::run(w[c], lds, t, xl, tw);
+            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, true>::run(w[c], lds, t, xl, tw);
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -322,36 +343,35 @@ __global__ __launch_bounds__(TP * PX) void k_zpass(cpx<R>* __restrict__ W2, int 
 // Forward z pass without the multiply (operator setup): split S3 -> 2 x B3
 // and transform, in place on W2 (L1, L2, Z, 3) per right-hand side.
 // --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PX>
-__global__ __launch_bounds__(TP * PX) void k_fwd_z(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int Z,
+template <typename R, int N, int TP, int PX, int MINW>
+__global__ __launch_bounds__(TP * PX, MINW) void k_fwd_z(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int Z,
                                                    int64_t w2s, const cpx<R>* __restrict__ phz,
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[PX * N];
+    __shared__ cpx<R> lds[2 * PX * N];
     const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
     const int m1 = blockIdx.x * PX + xl;
     const bool act = m1 < L1;
     const int m2 = blockIdx.y % L2, c = blockIdx.y / L2;
     const int64_t zs = (int64_t)L2 * L1;
     cpx<R>* col = W2 + (int64_t)blockIdx.z * w2s + (int64_t)c * Z * zs + (int64_t)m2 * L1 + m1;
-    cpx<R> ve[E], vo[E];
+    cpx<R> v[2][E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
         cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
         if (act && n < S3) a = col[(int64_t)n * zs];
         if (act && n + N < S3) b = col[(int64_t)(n + N) * zs];
-        ve[e] = a + b;
-        vo[e] = phz[n] * (a - b);
+        v[0][e] = a + b;
+        v[1][e] = phz[n] * (a - b);
     }
-    PencilFft<R, N, TP, PX, 1, false>::run(ve, lds, t, xl, tw);
-    PencilFft<R, N, TP, PX, 1, false>::run(vo, lds, t, xl, tw);
+    PencilFft<R, N, TP, PX, 1, false, 0, 2, PX * N>::run(v, lds, t, xl, tw);
     if (!act) return;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        col[(int64_t)(t + TP * e) * zs] = ve[e];
-        col[(int64_t)(N + t + TP * e) * zs] = vo[e];
+        col[(int64_t)(t + TP * e) * zs] = v[0][e];
+        col[(int64_t)(N + t + TP * e) * zs] = v[1][e];
     }
 }
 

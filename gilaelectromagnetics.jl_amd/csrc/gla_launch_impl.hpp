@@ -3,38 +3,71 @@
 #pragma once
 #include "gla_launch.hpp"
 
+// kernel-shape defaults (overridable at compile time for variant timing)
+#ifndef GLA_EROW
+#define GLA_EROW 8
+#endif
+#ifndef GLA_EZ_F
+#define GLA_EZ_F 8
+#endif
+#ifndef GLA_EZ_D
+#define GLA_EZ_D 4
+#endif
+#ifndef GLA_ZRELOAD
+#define GLA_ZRELOAD 0
+#endif
+#ifndef GLA_ZBATCH_KB
+#define GLA_ZBATCH_KB 64
+#endif
+
 namespace gla {
 
 // elements per thread: the divisor of N closest to `target` from below,
 // raised until the workgroup fits in `maxtp` threads per pencil
-__host__ __device__ constexpr int pick_e(int N, int target, int maxtp)
+// LDS exchanges of the radix plan for N with E elements per thread
+__host__ __device__ constexpr int plan_exchanges(int N, int E)
 {
-    int e = 1;
-    for (int d = 1; d <= N && d <= target; ++d)
-        if (N % d == 0) e = d;
-    while (N / e > maxtp) {
-        int nxt = e + 1;
-        while (N % nxt) ++nxt;
-        e = nxt;
+    const Plan p = make_plan(N, E);
+    if (p.nst <= 1) return (p.nst == 1 && E % p.rad[0] == 0) ? 0 : 1;
+    const bool fr = E % p.rad[0] == 0, lr = E % p.rad[p.nst - 1] == 0;
+    return (p.nst - 1) + (fr ? 0 : 1) + (lr ? 0 : 1);
+}
+
+// elements per thread: close to `target`, with few LDS exchanges, threads
+// per pencil within `maxtp` and, when `wave` is set, dividing 64 (a pencil
+// inside one wave)
+__host__ __device__ constexpr int pick_e(int N, int target, int maxtp, bool wave = false)
+{
+    int best = 0, bestscore = 1 << 30;
+    for (int e = 1; e <= N; ++e) {
+        if (N % e) continue;
+        const int tp = N / e;
+        if (tp > maxtp) continue;
+        const int d = e > target ? 2 * (e - target) : target - e;
+        const int score = d + 4 * plan_exchanges(N, e) + ((wave && 64 % tp != 0) ? 1000 : 0);
+        if (score < bestscore) { bestscore = score; best = e; }
     }
-    return e;
+    return best ? best : N;
 }
 
 template <typename R>
 struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
     static constexpr int PX = DBL ? 8 : 16;           // 128-byte rows across a bundle
-    static constexpr int E_ROW = 16;                  // passes 1, 2, 4, 5
-    static constexpr int E_Z = DBL ? 4 : 8;           // pass 3 (3 components live)
+    static constexpr int E_ROW = GLA_EROW;            // passes 1, 2, 4, 5
+    static constexpr int E_Z = DBL ? GLA_EZ_D : GLA_EZ_F;   // pass 3 (3 components live)
 };
 
 template <typename R, int N>
 struct LenK {
     using C = Cfg<R>;
-    static constexpr int TP_ROW = N / pick_e(N, C::E_ROW, 256);
+    static constexpr int TP_ROW = N / pick_e(N, C::E_ROW, 64, true);
     static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
     static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / C::PX);
     static constexpr int TP_Z = N / pick_e(N, C::E_Z, 1024 / C::PX);
+    // pass 3 runs its three component transforms together when their LDS
+    // buffers fit the budget
+    static constexpr int KB_Z = 3 * C::PX * N * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024 ? 3 : 1;
 };
 
 template <typename R, int N>
@@ -43,7 +76,7 @@ static hipError_t launch_fwd_x(const Geom& g, const DevTab& tb, const void* X, v
     using K = LenK<R, N>;
     const int rows = g.S[1] * g.S[2] * 3;
     dim3 grid((rows + K::PEN - 1) / K::PEN, nrhs);
-    hipLaunchKernelGGL((k_fwd_x<R, N, K::TP_ROW, K::PEN>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
+    hipLaunchKernelGGL((k_fwd_x<R, N, K::TP_ROW, K::PEN, 1>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
                        (const cpx<R>*)X, (cpx<R>*)W1, g.S[0], g.S[1], g.S[2], g.Y1, g.Z, g.xs, g.w1s,
                        (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0]);
     return hipGetLastError();
@@ -56,7 +89,7 @@ static hipError_t launch_inv_x(const Geom& g, const DevTab& tb, const void* W1, 
     using K = LenK<R, N>;
     const int rows = g.T[1] * g.T[2] * 3;
     dim3 grid((rows + K::PEN - 1) / K::PEN, nrhs);
-    hipLaunchKernelGGL((k_inv_x<R, N, K::TP_ROW, K::PEN>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
+    hipLaunchKernelGGL((k_inv_x<R, N, K::TP_ROW, K::PEN, 1>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
                        (const cpx<R>*)W1, (cpx<R>*)Y, g.T[0], g.T[1], g.T[2], g.Y1, g.Z, g.w1s, g.ys,
                        (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0], al, be, axpby);
     return hipGetLastError();
@@ -68,7 +101,7 @@ static hipError_t launch_fwd_y(const Geom& g, const DevTab& tb, const void* W1, 
     using K = LenK<R, N>;
     constexpr int PX = Cfg<R>::PX;
     dim3 grid((g.L[0] + PX - 1) / PX, g.S[2] * 3, nrhs);
-    hipLaunchKernelGGL((k_fwd_y<R, N, K::TP_Y, PX>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W1,
+    hipLaunchKernelGGL((k_fwd_y<R, N, K::TP_Y, PX, 1>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W1,
                        (cpx<R>*)W2, g.L[0], g.S[1], g.S[2], g.Y1, g.Z, g.w1s, g.w2s,
                        (const cpx<R>*)tb.phz[1], (const cpx<R>*)tb.tw[1]);
     return hipGetLastError();
@@ -80,7 +113,7 @@ static hipError_t launch_inv_y(const Geom& g, const DevTab& tb, const void* W2, 
     using K = LenK<R, N>;
     constexpr int PX = Cfg<R>::PX;
     dim3 grid((g.L[0] + PX - 1) / PX, g.T[2] * 3, nrhs);
-    hipLaunchKernelGGL((k_inv_y<R, N, K::TP_Y, PX>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W2,
+    hipLaunchKernelGGL((k_inv_y<R, N, K::TP_Y, PX, 1>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W2,
                        (cpx<R>*)W1, g.L[0], g.T[1], g.T[2], g.Y1, g.Z, g.w2s, g.w1s,
                        (const cpx<R>*)tb.phz[1], (const cpx<R>*)tb.tw[1]);
     return hipGetLastError();
@@ -97,9 +130,9 @@ static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nr
 #define GLA_ZARGS                                                                                   \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
         tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2]
-    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0>), grid, blk, 0, st, GLA_ZARGS);
-    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1>), grid, blk, 0, st, GLA_ZARGS);
-    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2>), grid, blk, 0, st, GLA_ZARGS);
+    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
+    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
+    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
 #undef GLA_ZARGS
     return hipGetLastError();
 }
@@ -110,7 +143,7 @@ static hipError_t launch_fwd_z(const Geom& g, const DevTab& tb, void* W2, int nr
     using K = LenK<R, N>;
     constexpr int PX = Cfg<R>::PX;
     dim3 grid((g.L[0] + PX - 1) / PX, g.L[1] * 3, nrhs);
-    hipLaunchKernelGGL((k_fwd_z<R, N, K::TP_Y, PX>), grid, dim3(K::TP_Y * PX), 0, st, (cpx<R>*)W2, g.L[0], g.L[1],
+    hipLaunchKernelGGL((k_fwd_z<R, N, K::TP_Y, PX, 1>), grid, dim3(K::TP_Y * PX), 0, st, (cpx<R>*)W2, g.L[0], g.L[1],
                        g.S[2], g.Z, g.w2s, (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2]);
     return hipGetLastError();
 }
