@@ -52,12 +52,14 @@ struct DevData {
     int device = 0;
     void* gf = nullptr;
     void* tab = nullptr;
+    void* order = nullptr;
     ~DevData()
     {
-        if (gf || tab) {
+        if (gf || tab || order) {
             DevGuard g(device);
             if (gf) (void)hipFree(gf);
             if (tab) (void)hipFree(tab);
+            if (order) (void)hipFree(order);
         }
     }
 };
@@ -206,16 +208,7 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
             tb[(size_t)(p++)] = C((R)std::cos(t), (R)std::sin(t));
         }
     }
-    auto dev = std::make_shared<DevData>();
-    dev->device = h->opts.device;
-    GLA_HIP(hipMalloc(&dev->gf, g.size() * sizeof(C)));
-    GLA_HIP(hipMalloc(&dev->tab, tb.size() * sizeof(C)));
-    GLA_HIP(hipMemcpy(dev->gf, g.data(), g.size() * sizeof(C), hipMemcpyHostToDevice));
-    GLA_HIP(hipMemcpy(dev->tab, tb.data(), tb.size() * sizeof(C), hipMemcpyHostToDevice));
-    h->dev = dev;
     for (int a = 0; a < 3; ++a) {
-        h->tab.phz[a] = (const C*)dev->tab + off[a][0];
-        h->tab.tw[a] = (const C*)dev->tab + off[a][1];
         gla::AxisMap& m = h->tab.am[a];
         m.B = (int)d.brn[a];
         m.U = (int)d.fur_ext[a];
@@ -224,7 +217,25 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
         m.fe = (T + 1) / 2 + (T % 2 == 0 ? 1 : 0);
         m.fo = (T + 1) / 2;
     }
+    const int Bv[3] = {(int)d.brn[0], (int)d.brn[1], (int)d.brn[2]};
+    const int Lv[3] = {2 * Bv[0], 2 * Bv[1], 2 * Bv[2]};
+    const std::vector<int> order = gla::zpass_order(Bv, Lv, h->tab.am, gla::zpass_tile_width((int)sizeof(C)));
+    auto dev = std::make_shared<DevData>();
+    dev->device = h->opts.device;
+    GLA_HIP(hipMalloc(&dev->gf, g.size() * sizeof(C)));
+    GLA_HIP(hipMalloc(&dev->tab, tb.size() * sizeof(C)));
+    GLA_HIP(hipMalloc(&dev->order, order.size() * sizeof(int)));
+    GLA_HIP(hipMemcpy(dev->gf, g.data(), g.size() * sizeof(C), hipMemcpyHostToDevice));
+    GLA_HIP(hipMemcpy(dev->tab, tb.data(), tb.size() * sizeof(C), hipMemcpyHostToDevice));
+    GLA_HIP(hipMemcpy(dev->order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice));
+    h->dev = dev;
+    for (int a = 0; a < 3; ++a) {
+        h->tab.phz[a] = (const C*)dev->tab + off[a][0];
+        h->tab.tw[a] = (const C*)dev->tab + off[a][1];
+    }
     h->tab.gf = dev->gf;
+    h->tab.zorder = (const int*)dev->order;
+    h->tab.zper_xcd = (int)(order.size() / 8);
     return GLA_OK;
 }
 
@@ -531,6 +542,42 @@ bool len_supported(int n)
     GLA_FOR_EACH_LEN(GLA_LEN_EQ)
 #undef GLA_LEN_EQ
     return false;
+}
+
+int zpass_tile_width(int complex_bytes) { return complex_bytes == 16 ? 8 : 16; }
+
+std::vector<int> zpass_order(const int B[3], const int L[3], const AxisMap am[3], int px)
+{
+    const int ntile = (L[0] + px - 1) / px;
+    const int nreal = ntile * L[1];
+    std::vector<int> ord;
+    ord.reserve((size_t)nreal + 8);
+    const bool mirror = am[0].sym && am[1].sym && B[0] % px == 0;
+    if (!mirror) {
+        for (int l = 0; l < nreal; ++l) ord.push_back((l % ntile) | ((l / ntile) << 16));
+    } else {
+        const int nt1 = B[0] / px;   // tiles per axis-1 parity
+        std::vector<char> done((size_t)nreal, 0);
+        auto emit = [&](int tile, int m2) {
+            const int id = m2 * ntile + tile;
+            if (!done[(size_t)id]) { done[(size_t)id] = 1; ord.push_back(tile | (m2 << 16)); }
+        };
+        for (int p2 = 0; p2 < 2; ++p2)
+            for (int k2 = 0; k2 < B[1]; ++k2) {
+                const int k2m = p2 == 0 ? (B[1] - k2) % B[1] : B[1] - 1 - k2;
+                if (k2m < k2) continue;
+                for (int p1 = 0; p1 < 2; ++p1)
+                    for (int a = 0; a < nt1; ++a) {
+                        const int am_ = nt1 - 1 - a;
+                        if (am_ < a) continue;
+                        const int t0 = p1 * nt1 + a, t1 = p1 * nt1 + am_;
+                        const int r0 = p2 * B[1] + k2, r1 = p2 * B[1] + k2m;
+                        emit(t0, r0); emit(t1, r0); emit(t0, r1); emit(t1, r1);
+                    }
+            }
+    }
+    while (ord.size() % 8) ord.push_back(-1);
+    return ord;
 }
 
 int len_next_supported(int n)

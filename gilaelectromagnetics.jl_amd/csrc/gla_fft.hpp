@@ -268,8 +268,16 @@ __device__ __forceinline__ void lds_sync()
 // of the workgroup (SYNC 0) or of the wave (SYNC 1) must call with the same
 // template arguments.  A masked pencil beyond the data still takes part.
 // ---------------------------------------------------------------------------
-template <typename R, int N, int TP, int NS, int PS, bool INV, int SYNC = 0, int K = 1, int KS = 0>
+template <typename R, int N, int TP, int NS, int PS, bool INV, int SYNC = 0, int K = 1, int KS = 0, int PADSH = 0>
 struct PencilFft {
+    // LDS slot of position pos: with PADSH > 0 one slot is skipped every
+    // 2^PADSH positions, so the stride-RAD writes of a stage spread over the
+    // banks (used when positions of one pencil are contiguous, NS == 1)
+    static __device__ __forceinline__ int slot(int pos)
+    {
+        if constexpr (PADSH > 0) return pos + (pos >> PADSH);
+        return pos;
+    }
     static constexpr int E = N / TP;
     static_assert(N % TP == 0, "TP must divide N");
     static constexpr Plan P = make_plan(N, E);
@@ -294,7 +302,7 @@ struct PencilFft {
 #pragma unroll
                 for (int k = 0; k < K; ++k)
 #pragma unroll
-                    for (int q = 0; q < RAD; ++q) a[k][u][q] = lds[k * KS + (j + q * NB) * NS + pen * PS];
+                    for (int q = 0; q < RAD; ++q) a[k][u][q] = lds[k * KS + slot(j + q * NB) * NS + pen * PS];
             }
         }
         lds_sync<SYNC>();
@@ -316,7 +324,7 @@ struct PencilFft {
                 for (int k = 0; k < K; ++k) {
                     dft<R, INV, RAD, N>(a[k][u], tw);
 #pragma unroll
-                    for (int q = 0; q < RAD; ++q) lds[k * KS + (base + q * NSP) * NS + pen * PS] = a[k][u][q];
+                    for (int q = 0; q < RAD; ++q) lds[k * KS + slot(base + q * NSP) * NS + pen * PS] = a[k][u][q];
                 }
             }
         }
@@ -357,7 +365,7 @@ struct PencilFft {
                         dft<R, INV, RAD, N>(a, tw);
                         const int j = t + TP * u;    // butterfly index, span 1
 #pragma unroll
-                        for (int q = 0; q < RAD; ++q) lds[k * KS + (j * RAD + q) * NS + pen * PS] = a[q];
+                        for (int q = 0; q < RAD; ++q) lds[k * KS + slot(j * RAD + q) * NS + pen * PS] = a[q];
                     }
                 lds_sync<SYNC>();
                 mid_stages<1>(lds, t, pen, tw);
@@ -365,7 +373,7 @@ struct PencilFft {
 #pragma unroll
                 for (int k = 0; k < K; ++k)
 #pragma unroll
-                    for (int e = 0; e < E; ++e) lds[k * KS + (t + TP * e) * NS + pen * PS] = v[k][e];
+                    for (int e = 0; e < E; ++e) lds[k * KS + slot(t + TP * e) * NS + pen * PS] = v[k][e];
                 lds_sync<SYNC>();
                 mid_stages<0>(lds, t, pen, tw);
             }
@@ -382,7 +390,7 @@ struct PencilFft {
 #pragma unroll
                     for (int k = 0; k < K; ++k)
 #pragma unroll
-                        for (int q = 0; q < RAD; ++q) a[k][q] = lds[k * KS + (j + q * NSP) * NS + pen * PS];
+                        for (int q = 0; q < RAD; ++q) a[k][q] = lds[k * KS + slot(j + q * NSP) * NS + pen * PS];
 #pragma unroll
                     for (int q = 1; q < RAD; ++q) {
                         const cpx<R> w = tw[q * j];
@@ -400,7 +408,7 @@ struct PencilFft {
 #pragma unroll
                 for (int k = 0; k < K; ++k)
 #pragma unroll
-                    for (int e = 0; e < E; ++e) v[k][e] = lds[k * KS + (t + TP * e) * NS + pen * PS];
+                    for (int e = 0; e < E; ++e) v[k][e] = lds[k * KS + slot(t + TP * e) * NS + pen * PS];
             }
             lds_sync<SYNC>();   // LDS free for the next transform
         }

@@ -52,7 +52,9 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
                                                           const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    constexpr int PS = N + 1;
+    // one padding slot every 8 positions (stride-8 stage writes hit distinct banks)
+    constexpr int PADSH = 3;
+    constexpr int PS = N + (N >> PADSH) + 1;
     // rows are whole pencils inside one wave when TP divides 64
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
     __shared__ cpx<R> lds[2 * PEN * PS];
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
         v[0][e] = a + b;
         v[1][e] = phz[n] * (a - b);
     }
-    PencilFft<R, N, TP, 1, PS, false, XSYNC, 2, PEN * PS>::run(v, lds, t, pen, tw);
+    PencilFft<R, N, TP, 1, PS, false, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
     if (act) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -95,7 +97,8 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
                                                           cpx<R> alpha, cpx<R> beta, int axpby)
 {
     constexpr int E = N / TP;
-    constexpr int PS = N + 1;
+    constexpr int PADSH = 3;
+    constexpr int PS = N + (N >> PADSH) + 1;
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
     __shared__ cpx<R> lds[2 * PEN * PS];
     const int t = threadIdx.x % TP, pen = threadIdx.x / TP;
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
         v[0][e] = act ? wr[k] : cpx<R>{R(0), R(0)};
         v[1][e] = act ? wr[N + k] : cpx<R>{R(0), R(0)};
     }
-    PencilFft<R, N, TP, 1, PS, true, XSYNC, 2, PEN * PS>::run(v, lds, t, pen, tw);
+    PencilFft<R, N, TP, 1, PS, true, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
     if (!act) return;
     const int top = T1 < N ? T1 : N;
     const bool rd = axpby && !(beta.x == R(0) && beta.y == R(0));
@@ -222,13 +225,22 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                                                    int Z, int64_t w2s, const cpx<R>* __restrict__ gf,
                                                    AxisMap a1, AxisMap a2, AxisMap a3,
                                                    const cpx<R>* __restrict__ phz,
-                                                   const cpx<R>* __restrict__ tw)
+                                                   const cpx<R>* __restrict__ tw,
+                                                   const int* __restrict__ order, int per_xcd)
 {
     constexpr int E = N / TP;
     __shared__ cpx<R> lds[KB * PX * N];
     const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
-    const int m1 = blockIdx.x * PX + xl;
-    const int m2 = blockIdx.y;
+    // workgroup h runs the column tile order[l]: the table places tiles that
+    // share stored G^ entries (mirror images on symmetric axes) next to each
+    // other in l, and l is chosen so that consecutive l run on one XCD
+    // (workgroups are dealt round-robin over the 8 XCDs)
+    const int h = blockIdx.x;
+    const int l = (h % 8) * per_xcd + h / 8;
+    const int tile = order[l];
+    if (tile < 0) return;   // padding entry (whole workgroup)
+    const int m1 = (tile & 0xffff) * PX + xl;
+    const int m2 = tile >> 16;
     const bool act = m1 < L1;
     const int64_t cs = (int64_t)Z * L2 * L1;      // component stride
     cpx<R>* col = W2 + (int64_t)blockIdx.z * w2s + (int64_t)m2 * L1 + m1;

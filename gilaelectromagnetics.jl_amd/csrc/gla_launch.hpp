@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <vector>
 #include "gla_kernels.hpp"
 
 // Branch lengths B (per-axis FFT lengths) compiled into the device library.
@@ -25,7 +27,18 @@ struct DevTab {
     const void* tw[3];     // e^{-2 pi i n / B_d}, n < B_d
     AxisMap am[3];
     const void* gf;        // G^ in device layout
+    const int* zorder;     // pass-3 tile order (tile | m2 << 16, -1 padding)
+    int zper_xcd;          // pass-3 workgroups per XCD (grid = 8 * zper_xcd)
 };
+
+// columns per pass-3 workgroup for the element type (bytes per complex)
+int zpass_tile_width(int complex_bytes);
+
+// Pass-3 tile order: ntile = ceil(L1 / PX) column tiles times L2 rows,
+// padded to a multiple of 8 with -1.  With symmetric storage on axes 1 and 2
+// the tiles whose stored Green entries coincide (mirror images) are listed
+// next to each other.
+std::vector<int> zpass_order(const int B[3], const int L[3], const AxisMap am[3], int px);
 
 bool len_supported(int n);
 int len_next_supported(int n);   // smallest supported length >= n, or 0

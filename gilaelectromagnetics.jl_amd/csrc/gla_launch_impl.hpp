@@ -58,6 +58,8 @@ struct Cfg {
     static constexpr int E_Z = DBL ? GLA_EZ_D : GLA_EZ_F;   // pass 3 (3 components live)
 };
 
+static_assert(Cfg<float>::PX == 16 && Cfg<double>::PX == 8, "zpass_tile_width() mirrors these widths");
+
 template <typename R, int N>
 struct LenK {
     using C = Cfg<R>;
@@ -124,12 +126,12 @@ static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nr
 {
     using K = LenK<R, N>;
     constexpr int PX = Cfg<R>::PX;
-    dim3 grid((g.L[0] + PX - 1) / PX, g.L[1], nrhs);
+    dim3 grid(8 * tb.zper_xcd, 1, nrhs);
     dim3 blk(K::TP_Z * PX);
     const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);
 #define GLA_ZARGS                                                                                   \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
-        tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2]
+        tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd
     if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
     else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
     else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);

@@ -198,3 +198,21 @@ def test_facepair_components_and_signs(orc):
     assert abs(g[0, 1]) > 1e-3 * abs(g[0, 0])
     assert abs(g[0, 2]) < 1e-12 * abs(g[0, 1]) and abs(g[1, 2]) < 1e-12 * abs(g[0, 1])
     assert abs(g[0, 0] - g[1, 1]) < 1e-10 * abs(g[0, 0])
+
+
+def test_golden_fixtures(orc):
+    # regression pins written by tests/golden/make_golden.py
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    g = np.load(os.path.join(here, "self_8.npz"))
+    N = tuple(int(v) for v in g["cells"])
+    toe = orc.toe_model(N, (float(g["h"]),) * 3, model=2, qn=10)
+    assert np.abs(toe - g["toe"]).max() <= 1e-12 * np.abs(g["toe"]).max()
+    fur = orc.fur_from_crc(N, N, (1, 1, 1), orc.crc_self(N, toe))
+    assert np.abs(fur - g["fur"]).max() <= 1e-12 * np.abs(g["fur"]).max()
+    assert orc.rel_l2(orc.matvec(N, N, (1, 1, 1), g["fur"], g["x"]), g["y"]) < 1e-14
+    e = np.load(os.path.join(here, "ext_4x6x2_6x8x4.npz"))
+    S, T = tuple(int(v) for v in e["src"]), tuple(int(v) for v in e["trg"])
+    assert orc.rel_l2(orc.matvec(S, T, (0, 0, 0), e["fur"], e["x"]), e["y"]) < 1e-14
+    crc = orc.crc_ext(S, T, (float(e["h"]),) * 3, tuple(e["delta"]), model=0)
+    assert orc.rel_l2(orc.brute(S, T, crc, e["x"]), e["y"]) < 1e-13
