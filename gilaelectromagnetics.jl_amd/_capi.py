@@ -28,6 +28,7 @@ EXPORTED = (
     "gla_last_error", "gla_init", "gla_opr_dims_for", "gla_opr_create_from_fur", "gla_opr_create",
     "gla_opr_export_fur", "gla_opr_get_dims", "gla_opr_adjoint", "gla_opr_destroy", "gla_opr_apply",
     "gla_opr_apply_axpby", "gla_opr_apply_host", "gla_opr_workspace_bytes", "gla_opr_profile",
+    "gla_ls_apply",
 )
 
 
@@ -82,6 +83,7 @@ def load(path=LIB_PATH):
         "gla_opr_apply_axpby": [vp, vp, i64, vp, i64, i64, d, d, d, d, vp],
         "gla_opr_apply_host": [vp, vp, i64, vp, i64, i64],
         "gla_opr_profile": [vp, vp, i64, vp, i64, i64, vp, i32, P(d), P(i64)],
+        "gla_ls_apply": [vp, vp, i64, i64, vp, i64, vp, i64, i64, vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

@@ -1,8 +1,8 @@
 // ComplexF64 instantiation of the matvec passes.
 #include "gla_launch_impl.hpp"
 template hipError_t gla::run_matvec<double>(const gla::Geom&, const gla::DevTab&, const void*, void*, void*, void*,
-                                            int, double, double, double, double, int, hipStream_t);
+                                            int, double, double, double, double, int, const gla::Epi&, hipStream_t);
 template hipError_t gla::run_pass<double>(int, const gla::Geom&, const gla::DevTab&, const void*, void*, void*, void*,
-                                      int, double, double, double, double, int, hipStream_t);
+                                      int, double, double, double, double, int, const gla::Epi&, hipStream_t);
 template hipError_t gla::run_fwd3<double>(const gla::Geom&, const gla::DevTab&, const void*, void*, void*, int,
                                           hipStream_t);

@@ -148,15 +148,20 @@ gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* ou
             return fail(GLA_ERR_SIZE, "the sum of source and target cells must be even on every axis "
                                       "(regenerate the volumes with an even cell count first)");
         d.brn[a] = (d.trg_cel[a] + d.src_cel[a]) / 2;
+        // a branch length without a compiled transform is embedded into the
+        // next compiled one (the circulant only needs L >= S + T - 1); such an
+        // axis stores its full branches
+        d.dev_brn[a] = gla::len_supported((int)d.brn[a]) ? d.brn[a] : gla::len_next_supported((int)d.brn[a]);
+        if (d.dev_brn[a] == 0)
+            return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) + " exceeds the compiled transforms");
         const bool same_org = trg->org_num[a] * sv.org_den[a] == sv.org_num[a] * trg->org_den[a];
-        d.sym[a] = (d.trg_cel[a] == d.src_cel[a] && same_org) ? 1 : 0;
+        d.sym[a] = (d.trg_cel[a] == d.src_cel[a] && same_org && d.dev_brn[a] == d.brn[a]) ? 1 : 0;
         if (d.sym[a]) {
             int64_t u = (d.trg_cel[a] + 1) / 2 + (d.trg_cel[a] % 2 == 0);
             d.fur_ext[a] = u < 2 ? 2 : u;
         } else {
-            d.fur_ext[a] = d.brn[a];
+            d.fur_ext[a] = d.dev_brn[a];
         }
-        d.dev_brn[a] = d.brn[a];
     }
     *out = d;
     return GLA_OK;
@@ -180,7 +185,7 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
     const gla_dims& d = h->dims;
     const int64_t U0 = d.fur_ext[0], U1 = d.fur_ext[1], U2 = d.fur_ext[2];
     const int64_t Ut = U0 * U1 * U2;
-    const double scale = 1.0 / (double)(8 * d.brn[0] * d.brn[1] * d.brn[2]);
+    const double scale = 1.0 / (double)(8 * d.dev_brn[0] * d.dev_brn[1] * d.dev_brn[2]);
     // device layout: ((((b*U2 + j2)*U1 + j1)*U0 + j0)*6 + comp
     std::vector<C> g((size_t)(8 * Ut * 6));
     for (int b = 0; b < 8; ++b)
@@ -191,12 +196,12 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
             }
     // phase and twiddle tables per axis
     int64_t ntab = 0;
-    for (int a = 0; a < 3; ++a) ntab += 2 * d.brn[a];
+    for (int a = 0; a < 3; ++a) ntab += 2 * d.dev_brn[a];
     std::vector<C> tb((size_t)ntab);
     int64_t off[3][2];
     int64_t p = 0;
     for (int a = 0; a < 3; ++a) {
-        const int64_t B = d.brn[a];
+        const int64_t B = d.dev_brn[a];
         off[a][0] = p;
         for (int64_t n = 0; n < B; ++n) {
             const double t = -M_PI * (double)n / (double)B;
@@ -210,14 +215,14 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
     }
     for (int a = 0; a < 3; ++a) {
         gla::AxisMap& m = h->tab.am[a];
-        m.B = (int)d.brn[a];
+        m.B = (int)d.dev_brn[a];
         m.U = (int)d.fur_ext[a];
         m.sym = (d.sym[a] && d.fur_ext[a] != d.brn[a]) ? 1 : 0;
         const int T = (int)d.trg_cel[a];
         m.fe = (T + 1) / 2 + (T % 2 == 0 ? 1 : 0);
         m.fo = (T + 1) / 2;
     }
-    const int Bv[3] = {(int)d.brn[0], (int)d.brn[1], (int)d.brn[2]};
+    const int Bv[3] = {(int)d.dev_brn[0], (int)d.dev_brn[1], (int)d.dev_brn[2]};
     const int Lv[3] = {2 * Bv[0], 2 * Bv[1], 2 * Bv[2]};
     const std::vector<int> order = gla::zpass_order(Bv, Lv, h->tab.am, gla::zpass_tile_width((int)sizeof(C)));
     auto dev = std::make_shared<DevData>();
@@ -273,7 +278,7 @@ gla_status ensure_ws(gla_opr_t h, int64_t nrhs)
 }
 
 gla_status run(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, double ar,
-               double ai, double br, double bi, int axpby, void* stream)
+               double ai, double br, double bi, int axpby, void* stream, const gla::Epi& epi = gla::Epi())
 {
     if (!h) return fail(GLA_ERR_ARG, "null operator");
     if (nrhs < 1 || nrhs > 65535) return fail(GLA_ERR_ARG, "nrhs must be in [1, 65535]");
@@ -288,10 +293,10 @@ gla_status run(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int6
     if (s) return s;
     hipError_t e;
     if (h->dims.dtype == GLA_C64)
-        e = gla::run_matvec<float>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby,
+        e = gla::run_matvec<float>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby, epi,
                                    (hipStream_t)stream);
     else
-        e = gla::run_matvec<double>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby,
+        e = gla::run_matvec<double>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby, epi,
                                     (hipStream_t)stream);
     if (e != hipSuccess) return fail(GLA_ERR_HIP, std::string("matvec launch: ") + hipGetErrorString(e));
     return GLA_OK;
@@ -307,6 +312,11 @@ bool finite_all(const T* p, int64_t n)
 
 }  // namespace
 
+namespace {
+gla_status create_common(const gla_opts* opts, const gla_vol* trg, const gla_vol* src, gla_dims d, const void* fur,
+                         int32_t fur_dtype, int64_t fur_len, gla_opr_t* out);
+}
+
 extern "C" {
 
 gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, const gla_vol* src,
@@ -321,6 +331,22 @@ gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, con
     if (s) return s;
     gla_dims d;
     if ((s = gla_opr_dims_for(trg, src, &d))) return s;
+    for (int a = 0; a < 3; ++a)
+        if (d.dev_brn[a] != d.brn[a])
+            return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) +
+                                                 " has no compiled transform; Fourier data in the reference layout "
+                                                 "cannot be re-embedded (build the operator with gla_opr_create)");
+    return create_common(opts, trg, src, d, fur, fur_dtype, fur_len, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+gla_status create_common(const gla_opts* opts, const gla_vol* trg, const gla_vol* src, gla_dims d, const void* fur,
+                         int32_t fur_dtype, int64_t fur_len, gla_opr_t* out)
+{
+    gla_status s;
     const int64_t Ut = d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2];
     if (fur_len != 8 * Ut * 6)
         return fail(GLA_ERR_SIZE, "Fourier Green data has " + std::to_string(fur_len) + " elements, expected 8*" +
@@ -328,10 +354,6 @@ gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, con
     const bool ok = fur_dtype == GLA_C64 ? finite_all((const float*)fur, fur_len)
                                          : finite_all((const double*)fur, fur_len);
     if (!ok) return fail(GLA_ERR_NONFINITE, "Fourier information contains non-numeric values.");
-    for (int a = 0; a < 3; ++a)
-        if (!gla::len_supported((int)d.brn[a]))
-            return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) +
-                                                 " has no compiled transform");
     d.dtype = opts->dtype;
     d.adj_mod = opts->adj_mod;
     std::vector<std::complex<double>> furd((size_t)fur_len);
@@ -367,6 +389,10 @@ gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, con
     return GLA_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vol* src, gla_opr_t* out)
 {
     if (!opts || !trg || !out) return fail(GLA_ERR_ARG, "null argument");
@@ -381,7 +407,7 @@ gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vo
     std::string err;
     s = gla::setup_fur(*opts, *trg, src ? *src : *trg, d, fur, err);
     if (s) return fail(s, err);
-    return gla_opr_create_from_fur(opts, trg, src, fur.data(), GLA_C128, (int64_t)fur.size(), out);
+    return create_common(opts, trg, src, d, fur.data(), GLA_C128, (int64_t)fur.size(), out);
 }
 
 gla_status gla_opr_export_fur(gla_opr_t h, void* buf, int64_t len)
@@ -429,6 +455,27 @@ gla_status gla_opr_apply_axpby(gla_opr_t h, const void* x, int64_t nx, void* y, 
                                double alpha_re, double alpha_im, double beta_re, double beta_im, void* stream)
 {
     return run(h, x, nx, y, ny, nrhs, alpha_re, alpha_im, beta_re, beta_im, 1, stream);
+}
+
+gla_status gla_ls_apply(gla_opr_t h, const void* chi, int64_t nchi, int64_t chi_stride, const void* x, int64_t nx,
+                        void* y, int64_t ny, int64_t nrhs, void* stream)
+{
+    if (!h || !chi) return fail(GLA_ERR_ARG, "null argument");
+    for (int a = 0; a < 3; ++a)
+        if (h->geom.S[a] != h->geom.T[a])
+            return fail(GLA_ERR_SIZE, "the Lippmann-Schwinger form needs equal source and target volumes");
+    const int64_t cells = (int64_t)h->geom.T[0] * h->geom.T[1] * h->geom.T[2];
+    if (chi_stride != 0 && chi_stride < cells) return fail(GLA_ERR_ARG, "chi_stride must be 0 or >= the cell count");
+    const int64_t need = chi_stride == 0 ? cells : chi_stride * (nrhs - 1) + cells;
+    if (nchi < need)
+        return fail(GLA_ERR_SIZE, "Green's operator and medium must have the same size.");
+    if (x == y) return fail(GLA_ERR_ARG, "x and y must not alias");
+    gla::Epi epi;
+    epi.x = x;
+    epi.xs = h->geom.xs;
+    epi.chi = chi;
+    epi.chis = chi_stride;
+    return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 2, stream, epi);
 }
 
 gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs)
@@ -504,8 +551,9 @@ gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, voi
         (void)hipEventRecord(e0, st);
         for (int r = 0; r < reps && e == hipSuccess; ++r)
             e = h->dims.dtype == GLA_C64
-                    ? gla::run_pass<float>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, st)
-                    : gla::run_pass<double>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, st);
+                    ? gla::run_pass<float>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(), st)
+                    : gla::run_pass<double>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(),
+                                            st);
         (void)hipEventRecord(e1, st);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float t = 0;

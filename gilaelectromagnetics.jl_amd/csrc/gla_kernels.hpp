@@ -94,8 +94,12 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
                                                           int64_t ws, int64_t ys,
                                                           const cpx<R>* __restrict__ phz,
                                                           const cpx<R>* __restrict__ tw,
-                                                          cpx<R> alpha, cpx<R> beta, int axpby)
+                                                          cpx<R> alpha, cpx<R> beta, int axpby,
+                                                          const cpx<R>* __restrict__ lsx, int64_t lsxs,
+                                                          const cpx<R>* __restrict__ chi, int64_t chis)
 {
+    // axpby 0: y = G x;  1: y = alpha G x + beta y;  2: y = x - chi (G x),
+    // chi one value per target cell (per right-hand side when chis > 0)
     constexpr int E = N / TP;
     constexpr int PADSH = 3;
     constexpr int PS = N + (N >> PADSH) + 1;
@@ -108,6 +112,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     const int i2 = act ? row % T2 : 0, i3 = act ? (row / T2) % T3 : 0, c = act ? row / (T2 * T3) : 0;
     const cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
     cpx<R>* yr = Y + (int64_t)blockIdx.y * ys + (int64_t)row * T1;
+    const int64_t cell0 = ((int64_t)i3 * T2 + i2) * T1;
     cpx<R> v[2][E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -118,21 +123,20 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     PencilFft<R, N, TP, 1, PS, true, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
     if (!act) return;
     const int top = T1 < N ? T1 : N;
-    const bool rd = axpby && !(beta.x == R(0) && beta.y == R(0));
+    const bool rd = axpby == 1 && !(beta.x == R(0) && beta.y == R(0));
+    const cpx<R>* xr = axpby == 2 ? lsx + (int64_t)blockIdx.y * lsxs + (int64_t)row * T1 : nullptr;
+    const cpx<R>* cr = axpby == 2 ? chi + (int64_t)blockIdx.y * chis + cell0 : nullptr;
+    auto epilogue = [&](cpx<R> r, int n) {
+        if (axpby == 1) r = alpha * r + (rd ? beta * yr[n] : cpx<R>{R(0), R(0)});
+        else if (axpby == 2) r = xr[n] - cr[n] * r;
+        yr[n] = r;
+    };
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
         const cpx<R> po = cmulc(v[1][e], phz[n]);
-        if (n < top) {
-            cpx<R> r = v[0][e] + po;
-            if (axpby) r = alpha * r + (rd ? beta * yr[n] : cpx<R>{R(0), R(0)});
-            yr[n] = r;
-        }
-        if (n + N < T1) {
-            cpx<R> r = v[0][e] - po;
-            if (axpby) r = alpha * r + (rd ? beta * yr[n + N] : cpx<R>{R(0), R(0)});
-            yr[n + N] = r;
-        }
+        if (n < top) epilogue(v[0][e] + po, n);
+        if (n + N < T1) epilogue(v[0][e] - po, n + N);
     }
 }
 

@@ -40,6 +40,14 @@ int zpass_tile_width(int complex_bytes);
 // next to each other.
 std::vector<int> zpass_order(const int B[3], const int L[3], const AxisMap am[3], int px);
 
+// last-pass epilogue operands for y = x - chi (G x) (axpby mode 2)
+struct Epi {
+    const void* x = nullptr;   // source field (self operator), per-RHS stride xs
+    int64_t xs = 0;
+    const void* chi = nullptr; // one value per target cell, per-RHS stride chis (0: shared)
+    int64_t chis = 0;
+};
+
 bool len_supported(int n);
 int len_next_supported(int n);   // smallest supported length >= n, or 0
 
@@ -49,7 +57,8 @@ constexpr int NPASS = 5;
 // 4 x-inverse
 template <typename R>
 hipError_t run_pass(int pass, const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2,
-                    int nrhs, double ar, double ai, double br, double bi, int axpby, hipStream_t st);
+                    int nrhs, double ar, double ai, double br, double bi, int axpby, const Epi& epi,
+                    hipStream_t st);
 
 // forward x, y, z passes only (operator setup): X (S1,S2,S3,3) -> W2 with
 // every axis split and transformed
@@ -58,7 +67,7 @@ hipError_t run_fwd3(const Geom& g, const DevTab& tb, const void* X, void* W1, vo
 
 template <typename R>
 hipError_t run_matvec(const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2,
-                      int nrhs, double ar, double ai, double br, double bi, int axpby,
+                      int nrhs, double ar, double ai, double br, double bi, int axpby, const Epi& epi,
                       hipStream_t st);
 
 }  // namespace gla

@@ -86,14 +86,15 @@ static hipError_t launch_fwd_x(const Geom& g, const DevTab& tb, const void* X, v
 
 template <typename R, int N>
 static hipError_t launch_inv_x(const Geom& g, const DevTab& tb, const void* W1, void* Y, int nrhs,
-                               cpx<R> al, cpx<R> be, int axpby, hipStream_t st)
+                               cpx<R> al, cpx<R> be, int axpby, const Epi& epi, hipStream_t st)
 {
     using K = LenK<R, N>;
     const int rows = g.T[1] * g.T[2] * 3;
     dim3 grid((rows + K::PEN - 1) / K::PEN, nrhs);
     hipLaunchKernelGGL((k_inv_x<R, N, K::TP_ROW, K::PEN, 1>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
                        (const cpx<R>*)W1, (cpx<R>*)Y, g.T[0], g.T[1], g.T[2], g.Y1, g.Z, g.w1s, g.ys,
-                       (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0], al, be, axpby);
+                       (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0], al, be, axpby,
+                       (const cpx<R>*)epi.x, epi.xs, (const cpx<R>*)epi.chi, epi.chis);
     return hipGetLastError();
 }
 
@@ -170,7 +171,8 @@ hipError_t run_fwd3(const Geom& g, const DevTab& tb, const void* X, void* W1, vo
 
 template <typename R>
 hipError_t run_pass(int pass, const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2,
-                    int nrhs, double ar, double ai, double br, double bi, int axpby, hipStream_t st)
+                    int nrhs, double ar, double ai, double br, double bi, int axpby, const Epi& epi,
+                    hipStream_t st)
 {
     hipError_t e = hipErrorInvalidValue;
     const cpx<R> al{R(ar), R(ai)}, be{R(br), R(bi)};
@@ -178,7 +180,7 @@ hipError_t run_pass(int pass, const Geom& g, const DevTab& tb, const void* X, vo
 #define GLA_CASE_2(n) case n: e = launch_fwd_y<R, n>(g, tb, W1, W2, nrhs, st); break;
 #define GLA_CASE_3(n) case n: e = launch_zpass<R, n>(g, tb, W2, nrhs, st); break;
 #define GLA_CASE_4(n) case n: e = launch_inv_y<R, n>(g, tb, W2, W1, nrhs, st); break;
-#define GLA_CASE_5(n) case n: e = launch_inv_x<R, n>(g, tb, W1, Y, nrhs, al, be, axpby, st); break;
+#define GLA_CASE_5(n) case n: e = launch_inv_x<R, n>(g, tb, W1, Y, nrhs, al, be, axpby, epi, st); break;
     switch (pass) {
         case 0: switch (g.B[0]) { GLA_FOR_EACH_LEN(GLA_CASE_1) default: break; } break;
         case 1: switch (g.B[1]) { GLA_FOR_EACH_LEN(GLA_CASE_2) default: break; } break;
@@ -197,10 +199,10 @@ hipError_t run_pass(int pass, const Geom& g, const DevTab& tb, const void* X, vo
 
 template <typename R>
 hipError_t run_matvec(const Geom& g, const DevTab& tb, const void* X, void* Y, void* W1, void* W2, int nrhs,
-                      double ar, double ai, double br, double bi, int axpby, hipStream_t st)
+                      double ar, double ai, double br, double bi, int axpby, const Epi& epi, hipStream_t st)
 {
     for (int p = 0; p < NPASS; ++p) {
-        hipError_t e = run_pass<R>(p, g, tb, X, Y, W1, W2, nrhs, ar, ai, br, bi, axpby, st);
+        hipError_t e = run_pass<R>(p, g, tb, X, Y, W1, W2, nrhs, ar, ai, br, bi, axpby, epi, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

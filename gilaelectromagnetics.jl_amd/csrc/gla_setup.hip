@@ -158,20 +158,22 @@ __global__ void k_toeplitz(double2* __restrict__ toe, int N1, int N2, int N3, Gr
     for (int k = 0; k < 6; ++k) toe[p * 6 + k] = make_double2(o[2 * k], o[2 * k + 1]);
 }
 
-// self circulant from the Toeplitz tensor: c[n] = toe[n] (n < N), 0 at N,
-// flips f_i f_j of toe[2N - n] beyond.  Output layout (L1,L2,L3, comp6).
-__global__ void k_embed_self(double2* __restrict__ crc, const double2* __restrict__ toe, int N1, int N2, int N3)
+// self circulant from the Toeplitz tensor on an L grid (L >= 2N - 1 per
+// axis): c[n] = toe[n] for n < N, zero for N <= n <= L - N, and the flips
+// f_i f_j of toe[L - n] beyond (L = 2N is the reference's egoToeCrc!).
+// Output layout (L1,L2,L3, comp6).
+__global__ void k_embed_self(double2* __restrict__ crc, const double2* __restrict__ toe, int N1, int N2, int N3,
+                             int L1, int L2, int L3)
 {
-    const int L1 = 2 * N1, L2 = 2 * N2, L3 = 2 * N3;
     const int64_t Lt = (int64_t)L1 * L2 * L3;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Lt) return;
     const int n1 = (int)(p % L1), n2 = (int)((p / L1) % L2), n3 = (int)(p / ((int64_t)L1 * L2));
-    if (n1 == N1 || n2 == N2 || n3 == N3) {
+    if ((n1 >= N1 && n1 <= L1 - N1) || (n2 >= N2 && n2 <= L2 - N2) || (n3 >= N3 && n3 <= L3 - N3)) {
         for (int k = 0; k < 6; ++k) crc[k * Lt + p] = make_double2(0, 0);
         return;
     }
-    const int q1 = n1 < N1 ? n1 : 2 * N1 - n1, q2 = n2 < N2 ? n2 : 2 * N2 - n2, q3 = n3 < N3 ? n3 : 2 * N3 - n3;
+    const int q1 = n1 < N1 ? n1 : L1 - n1, q2 = n2 < N2 ? n2 : L2 - n2, q3 = n3 < N3 ? n3 : L3 - n3;
     const double f1 = n1 < N1 ? 1 : -1, f2 = n2 < N2 ? 1 : -1, f3 = n3 < N3 ? 1 : -1;
     const double2* s = toe + ((int64_t)q1 + (int64_t)N1 * (q2 + (int64_t)N2 * q3)) * 6;
     const double sg[6] = {1, 1, 1, f1 * f2, f1 * f3, f2 * f3};
@@ -179,14 +181,15 @@ __global__ void k_embed_self(double2* __restrict__ crc, const double2* __restric
 }
 
 // external circulant over target-minus-source separations
-__global__ void k_crc_ext(double2* __restrict__ crc, int T1, int T2, int T3, int L1, int L2, int L3, double d1,
-                          double d2, double d3, GreenCfg G, const double* __restrict__ gl)
+// (zero for T <= n <= L - S on an axis; L = S + T is the reference's grid)
+__global__ void k_crc_ext(double2* __restrict__ crc, int T1, int T2, int T3, int S1, int S2, int S3, int L1, int L2,
+                          int L3, double d1, double d2, double d3, GreenCfg G, const double* __restrict__ gl)
 {
     const int64_t Lt = (int64_t)L1 * L2 * L3;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= Lt) return;
     const int n1 = (int)(p % L1), n2 = (int)((p / L1) % L2), n3 = (int)(p / ((int64_t)L1 * L2));
-    if (n1 == T1 || n2 == T2 || n3 == T3) {
+    if ((n1 >= T1 && n1 <= L1 - S1) || (n2 >= T2 && n2 <= L2 - S2) || (n3 >= T3 && n3 <= L3 - S3)) {
         for (int k = 0; k < 6; ++k) crc[k * Lt + p] = make_double2(0, 0);
         return;
     }
@@ -263,8 +266,8 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
         }                                                                          \
     } while (0)
     for (int a = 0; a < 3; ++a)
-        if (!len_supported((int)d.brn[a])) {
-            err = "branch length " + std::to_string(d.brn[a]) + " has no compiled transform";
+        if (!len_supported((int)d.dev_brn[a])) {
+            err = "branch length " + std::to_string(d.dev_brn[a]) + " has no compiled transform";
             return GLA_ERR_UNSUPPORTED;
         }
     GreenCfg G{};
@@ -310,7 +313,8 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
             gl[(size_t)((q * (q - 1) / 2 + i) * 2 + 1)] = w[i];
         }
     }
-    const int B[3] = {(int)d.brn[0], (int)d.brn[1], (int)d.brn[2]};
+    // transform lengths; B > (S+T)/2 embeds the circulant in a longer grid
+    const int B[3] = {(int)d.dev_brn[0], (int)d.dev_brn[1], (int)d.dev_brn[2]};
     const int L[3] = {2 * B[0], 2 * B[1], 2 * B[2]};
     const int64_t Lt = (int64_t)L[0] * L[1] * L[2];
     DevBuf dgl, dcrc, dw1, dw2, dtab, dtoe, dfur, dcnt;
@@ -325,7 +329,8 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
                            (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], G, (const double*)dgl.p);
         SETUP_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_embed_self, dim3((unsigned)((Lt + 255) / 256)), dim3(256), 0, nullptr, (double2*)dcrc.p,
-                           (const double2*)dtoe.p, (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2]);
+                           (const double2*)dtoe.p, (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], L[0], L[1],
+                           L[2]);
         SETUP_HIP(hipGetLastError());
     } else {
         // separation of the first target cell centre from the first source one
@@ -337,8 +342,9 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
             del[a] = t0 - s0;
         }
         hipLaunchKernelGGL(k_crc_ext, dim3((unsigned)((Lt + thr - 1) / thr)), dim3(thr), 0, nullptr, (double2*)dcrc.p,
-                           (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], L[0], L[1], L[2], del[0], del[1],
-                           del[2], G, (const double*)dgl.p);
+                           (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], (int)d.src_cel[0],
+                           (int)d.src_cel[1], (int)d.src_cel[2], L[0], L[1], L[2], del[0], del[1], del[2], G,
+                           (const double*)dgl.p);
         SETUP_HIP(hipGetLastError());
     }
     // forward transform over the full L grid: S = L on every axis

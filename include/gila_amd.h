@@ -149,6 +149,18 @@ gla_status gla_opr_apply_axpby(gla_opr_t h, const void* x, int64_t nx, void* y, 
                                double alpha_re, double alpha_im, double beta_re, double beta_im,
                                void* stream);
 
+/* Lippmann-Schwinger operator y = x - chi .* (G x) of the reference's
+ * examples (LippmannSchwinger, examples/GilaOperators.jl:57-96), with the
+ * medium multiply and the subtraction fused into the last kernel.  Self
+ * operators only.  chi: one complex value per cell, (T1,T2,T3), in the
+ * operator's dtype; chi_stride = 0 shares it across the nrhs right-hand
+ * sides, otherwise it is the element stride between per-RHS media; nchi is
+ * the number of elements of the chi buffer.  x and y must not alias.  No
+ * counterpart entry point in the reference (its LippmannSchwinger is user
+ * code over egoOpr!). */
+gla_status gla_ls_apply(gla_opr_t h, const void* chi, int64_t nchi, int64_t chi_stride, const void* x, int64_t nx,
+                        void* y, int64_t ny, int64_t nrhs, void* stream);
+
 /* G * x on host memory (src/glaLinAlg.jl:33-43 with the host-to-device copy
  * of :37-40): copies in, applies, copies out and synchronizes. */
 gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs);

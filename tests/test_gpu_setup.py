@@ -80,3 +80,27 @@ def test_setup_operator_matvec(gila, orc, gpu, typ):
     y = (opr * xd).cpu().numpy().reshape(N + (3,), order="F")
     y_ref = orc.matvec(N, N, (1, 1, 1), fur64, x)
     assert orc.rel_l2(y, y_ref) <= (1e-12 if typ == np.complex128 else 1e-5)
+
+
+def _first_centre(vol):
+    return [float(o) - float(s) * (c // 2 - (0.5 if c % 2 == 0 else 0.0)) for o, s, c in zip(vol.org, vol.scl, vol.cel)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ,tol", [(np.complex128, 1e-10), (np.complex64, 1e-5)])
+def test_contained_source_with_reembedded_axes(gila, orc, gpu, typ, tol):
+    # a source inside a larger target on the same grid is the target's self
+    # operator restricted to the source (the guide-in-vacuum semantics);
+    # here B = (S+T)/2 = (6, 7, 7) and the length-7 axes are embedded in 8
+    tgt = gila.GlaVol((8, 12, 10), (H,) * 3, (0, 0, 0))
+    src = gila.GlaVol((4, 2, 4), (H,) * 3, (0, 0, H))
+    ext = gila.GlaOprMem(gila.GlaKerOpt(False), tgt, src, setTyp=typ)
+    assert tuple(ext._dims.brn) == (6, 7, 7) and tuple(ext._dims.dev_brn) == (6, 8, 8)
+    slf = gila.GlaOprMem(gila.GlaKerOpt(False), tgt, setTyp=typ)
+    off = [round((a - b) / float(H)) for a, b in zip(_first_centre(src), _first_centre(tgt))]
+    x = orc.complex_gaussian(src.cel + (3,), seed=13).astype(typ)
+    xt = np.zeros(tgt.cel + (3,), typ, order="F")
+    xt[off[0]:off[0] + 4, off[1]:off[1] + 2, off[2]:off[2] + 4, :] = x
+    y_ext = gila.egoOpr_(ext, x)
+    y_slf = gila.egoOpr_(slf, xt)
+    assert orc.rel_l2(y_ext, y_slf) <= tol
