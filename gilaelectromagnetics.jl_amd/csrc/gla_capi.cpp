@@ -592,7 +592,7 @@ bool len_supported(int n)
     return false;
 }
 
-int zpass_tile_width(int complex_bytes) { return complex_bytes == 16 ? 8 : 16; }
+int zpass_tile_width(int complex_bytes) { return zpass_px(complex_bytes); }
 
 std::vector<int> zpass_order(const int B[3], const int L[3], const AxisMap am[3], int px)
 {

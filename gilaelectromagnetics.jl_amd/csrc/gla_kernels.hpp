@@ -18,6 +18,7 @@
 // Passes: 1 x-forward, 2 y-forward, 3 z-forward * G^ * z-inverse (in place),
 // 4 y-inverse, 5 x-inverse (+ alpha/beta epilogue).
 #pragma once
+#include <type_traits>
 #include "gla_fft.hpp"
 
 namespace gla {
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
 // G^ device layout: gf[(((b*U3 + j3)*U2 + j2)*U1 + j1)*6 + comp],
 // comp order [xx,yy,zz,xy,xz,yz], pre-scaled by 1/(L1 L2 L3).
 // --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PX, int MODE, int MINW, int KB, int RELOAD>
+template <typename R, int N, int TP, int PX, int MODE, int MINW, int KB, int RELOAD, int WL, int GG, int PADSH>
 __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int T3,
                                                    int Z, int64_t w2s, const cpx<R>* __restrict__ gf,
                                                    AxisMap a1, AxisMap a2, AxisMap a3,
@@ -233,8 +234,31 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                                                    const int* __restrict__ order, int per_xcd)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[KB * PX * N];
-    const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
+    // WL: every pencil lives inside one wave (64/TP columns per wave, each
+    // column's TP threads adjacent), in its own padded LDS row, so the
+    // exchanges between stages need no workgroup barrier
+    static_assert(!WL || (64 % TP == 0 && (TP * PX) % 64 == 0), "wave-local pencils need TP | 64");
+    constexpr int CPW = WL ? 64 / TP : 1;
+    constexpr int PSW = WL ? N + (N >> PADSH) + 1 : N;
+    __shared__ cpx<R> lds[KB * PX * PSW];
+    int xl, t;
+    if constexpr (WL) {
+        const int lane = threadIdx.x & 63;
+        xl = (threadIdx.x >> 6) * CPW + lane % CPW;
+        t = lane / CPW;
+    } else {
+        xl = threadIdx.x % PX;
+        t = threadIdx.x / PX;
+    }
+    // stride-interleaved pencils (block barrier) or padded pencil rows (wave sync)
+    using FftF3 = std::conditional_t<WL, PencilFft<R, N, TP, 1, PSW, false, 1, 3, PX * PSW, PADSH>,
+                                     PencilFft<R, N, TP, PX, 1, false, 0, 3, PX * N>>;
+    using FftI3 = std::conditional_t<WL, PencilFft<R, N, TP, 1, PSW, true, 1, 3, PX * PSW, PADSH>,
+                                     PencilFft<R, N, TP, PX, 1, true, 0, 3, PX * N>>;
+    using FftF1 = std::conditional_t<WL, PencilFft<R, N, TP, 1, PSW, false, 1, 1, 0, PADSH>,
+                                     PencilFft<R, N, TP, PX, 1, false>>;
+    using FftI1 = std::conditional_t<WL, PencilFft<R, N, TP, 1, PSW, true, 1, 1, 0, PADSH>,
+                                     PencilFft<R, N, TP, PX, 1, true>>;
     // workgroup h runs the column tile order[l]: the table places tiles that
     // share stored G^ entries (mirror images on symmetric axes) next to each
     // other in l, and l is chosen so that consecutive l run on one XCD
@@ -255,7 +279,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
     constexpr int XE = RELOAD ? 1 : E;
     cpx<R> xa[3][XE];
     cpx<R> xb[MODE == 1 ? 3 : 1][MODE == 1 ? XE : 1];
-    if constexpr (!RELOAD) {
+    if constexpr (RELOAD == 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -274,6 +298,84 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
     axis_lookup(a1, p1, act ? k1 : 0, j1, s1);
     axis_lookup(a2, p2, k2, j2, s2);
     const R s12 = R(s1 * s2);
+
+    // symmetric 3x3 multiply of branch (p1,p2,p3) (mulKerHst!, src/glaActSup.jl:879-899)
+    auto mul_g = [&](cpx<R>(&w)[3][E], int p3) {
+        const int b = (p1 << 2) | (p2 << 1) | p3;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            // G^ loads in groups of GG points: bounds the registers the
+            // scheduler spends on loads in flight
+            if constexpr (GG < E)
+                if (e > 0 && e % GG == 0) __builtin_amdgcn_sched_barrier(0);
+            const int k3 = t + TP * e;
+            int j3, s3;
+            axis_lookup(a3, p3, k3, j3, s3);
+            if (act) {
+                const cpx<R>* g = gf + ((((int64_t)b * a3.U + j3) * a2.U + j2) * a1.U + j1) * 6;
+                const cpx<R> gxx = g[0], gyy = g[1], gzz = g[2];
+                const R s13 = R(s1 * s3), s23 = R(s2 * s3);
+                const cpx<R> gxy = s12 * g[3], gxz = s13 * g[4], gyz = s23 * g[5];
+                const cpx<R> x0 = w[0][e], x1 = w[1][e], x2 = w[2][e];
+                w[0][e] = gxx * x0 + gxy * x1 + gxz * x2;
+                w[1][e] = gxy * x0 + gyy * x1 + gyz * x2;
+                w[2][e] = gxz * x0 + gyz * x1 + gzz * x2;
+            }
+        }
+    };
+    auto fwd = [&](cpx<R>(&w)[3][E]) {
+        if constexpr (KB == 3) {
+            FftF3::run(w, lds, t, xl, tw);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) FftF1::run(w[c], lds, t, xl, tw);
+        }
+    };
+    auto inv = [&](cpx<R>(&w)[3][E]) {
+        if constexpr (KB == 3) {
+            FftI3::run(w, lds, t, xl, tw);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) FftI1::run(w[c], lds, t, xl, tw);
+        }
+    };
+
+    if constexpr (RELOAD == 2) {
+        // both parities split first: the input dies at the split, so the
+        // live state is the two parities' fields only
+        cpx<R> wv[2][3][E];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int n = t + TP * e;
+                const cpx<R> a = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
+                cpx<R> b = {R(0), R(0)};
+                if constexpr (MODE == 1)
+                    b = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
+                wv[0][c][e] = a + b;
+                wv[1][c][e] = phz[n] * (a - b);
+            }
+        fwd(wv[0]);
+        mul_g(wv[0], 0);
+        inv(wv[0]);
+        fwd(wv[1]);
+        mul_g(wv[1], 1);
+        inv(wv[1]);
+        if (!act) return;
+        const int top = T3 < N ? T3 : N;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int n = t + TP * e;
+                const cpx<R> po = cmulc(wv[1][c][e], phz[n]);
+                if (n < top) col[c * cs + (int64_t)n * zs] = wv[0][c][e] + po;
+                if constexpr (MODE == 2)
+                    if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = wv[0][c][e] - po;
+            }
+        return;
+    }
 
     cpx<R> acc[3][E];
     cpx<R> accb[MODE == 2 ? 3 : 1][MODE == 2 ? E : 1];
@@ -297,36 +399,9 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                 }
                 w[c][e] = p3 == 0 ? a + b : phz[n] * (a - b);
             }
-        if constexpr (KB == 3) {
-            PencilFft<R, N, TP, PX, 1, false, 0, 3, PX * N>::run(w, lds, t, xl, tw);
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, false>::run(w[c], lds, t, xl, tw);
-        }
-        // symmetric 3x3 multiply (mulKerHst!, src/glaActSup.jl:879-899)
-        const int b = (p1 << 2) | (p2 << 1) | p3;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int k3 = t + TP * e;
-            int j3, s3;
-            axis_lookup(a3, p3, k3, j3, s3);
-            if (act) {
-                const cpx<R>* g = gf + ((((int64_t)b * a3.U + j3) * a2.U + j2) * a1.U + j1) * 6;
-                const cpx<R> gxx = g[0], gyy = g[1], gzz = g[2];
-                const R s13 = R(s1 * s3), s23 = R(s2 * s3);
-                const cpx<R> gxy = s12 * g[3], gxz = s13 * g[4], gyz = s23 * g[5];
-                const cpx<R> x0 = w[0][e], x1 = w[1][e], x2 = w[2][e];
-                w[0][e] = gxx * x0 + gxy * x1 + gxz * x2;
-                w[1][e] = gxy * x0 + gyy * x1 + gyz * x2;
-                w[2][e] = gxz * x0 + gyz * x1 + gzz * x2;
-            }
-        }
-        if constexpr (KB == 3) {
-            PencilFft<R, N, TP, PX, 1, true, 0, 3, PX * N>::run(w, lds, t, xl, tw);
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, true>::run(w[c], lds, t, xl, tw);
-        }
+        fwd(w);
+        mul_g(w, p3);
+        inv(w);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll

@@ -31,6 +31,15 @@ struct DevTab {
     int zper_xcd;          // pass-3 workgroups per XCD (grid = 8 * zper_xcd)
 };
 
+// columns per pass-3 workgroup (overridable at compile time for variant timing)
+#ifndef GLA_PXZ_F
+#define GLA_PXZ_F 16
+#endif
+#ifndef GLA_PXZ_D
+#define GLA_PXZ_D 8
+#endif
+constexpr int zpass_px(int complex_bytes) { return complex_bytes == 16 ? GLA_PXZ_D : GLA_PXZ_F; }
+
 // columns per pass-3 workgroup for the element type (bytes per complex)
 int zpass_tile_width(int complex_bytes);
 

@@ -13,11 +13,28 @@
 #ifndef GLA_EZ_D
 #define GLA_EZ_D 4
 #endif
-#ifndef GLA_ZRELOAD
-#define GLA_ZRELOAD 0
+// pass-3 schedule: 0 keeps the input column in registers through the even
+// parity, 2 splits both parities first (fewer registers for FP64)
+#ifndef GLA_ZRELOAD_F
+#define GLA_ZRELOAD_F 0
+#endif
+#ifndef GLA_ZRELOAD_D
+#define GLA_ZRELOAD_D 2
 #endif
 #ifndef GLA_ZBATCH_KB
 #define GLA_ZBATCH_KB 64
+#endif
+#ifndef GLA_ZWAVE
+#define GLA_ZWAVE 0
+#endif
+#ifndef GLA_ZMINW
+#define GLA_ZMINW 1
+#endif
+#ifndef GLA_ZGG
+#define GLA_ZGG 64
+#endif
+#ifndef GLA_ZPADSH
+#define GLA_ZPADSH 3
 #endif
 
 namespace gla {
@@ -56,9 +73,11 @@ struct Cfg {
     static constexpr int PX = DBL ? 8 : 16;           // 128-byte rows across a bundle
     static constexpr int E_ROW = GLA_EROW;            // passes 1, 2, 4, 5
     static constexpr int E_Z = DBL ? GLA_EZ_D : GLA_EZ_F;   // pass 3 (3 components live)
+    static constexpr int PXZ = zpass_px(sizeof(cpx<R>));       // pass-3 columns per workgroup
+    static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
 };
 
-static_assert(Cfg<float>::PX == 16 && Cfg<double>::PX == 8, "zpass_tile_width() mirrors these widths");
+static_assert(Cfg<float>::PXZ == zpass_px(8) && Cfg<double>::PXZ == zpass_px(16), "one pass-3 tile width");
 
 template <typename R, int N>
 struct LenK {
@@ -66,10 +85,13 @@ struct LenK {
     static constexpr int TP_ROW = N / pick_e(N, C::E_ROW, 64, true);
     static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
     static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / C::PX);
-    static constexpr int TP_Z = N / pick_e(N, C::E_Z, 1024 / C::PX);
+    static constexpr int TP_Z = N / pick_e(N, C::E_Z, 1024 / C::PXZ, GLA_ZWAVE != 0);
+    // pass 3 keeps each pencil inside one wave when its threads divide 64
+    static constexpr int WL_Z = (GLA_ZWAVE != 0 && 64 % TP_Z == 0 && (TP_Z * C::PXZ) % 64 == 0) ? 1 : 0;
+    static constexpr int PS_Z = WL_Z ? N + (N >> GLA_ZPADSH) + 1 : N;
     // pass 3 runs its three component transforms together when their LDS
     // buffers fit the budget
-    static constexpr int KB_Z = 3 * C::PX * N * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024 ? 3 : 1;
+    static constexpr int KB_Z = 3 * C::PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024 ? 3 : 1;
 };
 
 template <typename R, int N>
@@ -126,16 +148,17 @@ template <typename R, int N>
 static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
 {
     using K = LenK<R, N>;
-    constexpr int PX = Cfg<R>::PX;
+    constexpr int PX = Cfg<R>::PXZ;
+    constexpr int SC = Cfg<R>::ZSCHED;
     dim3 grid(8 * tb.zper_xcd, 1, nrhs);
     dim3 blk(K::TP_Z * PX);
     const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);
 #define GLA_ZARGS                                                                                   \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
         tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd
-    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
-    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
-    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, 1, K::KB_Z, GLA_ZRELOAD>), grid, blk, 0, st, GLA_ZARGS);
+    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
 #undef GLA_ZARGS
     return hipGetLastError();
 }

@@ -1,0 +1,161 @@
+"""Lippmann-Schwinger operator and the mixed-precision GMRES solver.
+
+The operator of examples/GilaOperators.jl:57-96 (A x = x - chi .* G x) is
+checked on the GPU against the oracle's matvec; the solver's host logic is
+checked on the CPU with operators backed by the oracle, and on the GPU by
+the residual of the FP64 oracle operator at the returned solution."""
+import numpy as np
+import pytest
+
+H = 1.0 / 32
+
+
+def _sphere_medium(N, chi, radius_frac=0.4):
+    c = [(np.arange(n) + 0.5) / n - 0.5 for n in N]
+    X, Y, Z = np.meshgrid(*c, indexing="ij")
+    inside = X ** 2 + Y ** 2 + Z ** 2 <= radius_frac ** 2
+    m = np.zeros(N, np.complex128, order="F")
+    m[inside] = chi
+    return m
+
+
+class OracleLS:
+    """A x = x - chi .* (G x) with the oracle's matvec; the interface the solver
+    uses (apply, restrict, n, nbatch)."""
+
+    def __init__(self, orc, N, fur, media, dtype):
+        self.orc, self.N, self.fur, self.dtype = orc, N, fur, dtype
+        self.media = [np.asarray(m, dtype) for m in media]
+        self.nbatch = len(self.media)
+        self.n = 3 * int(np.prod(N))
+
+    def apply(self, x, y=None):
+        import torch
+        xb = x.reshape(-1, self.n)
+        out = np.empty(tuple(xb.shape), self.dtype)
+        for r in range(xb.shape[0]):
+            v = xb[r].cpu().numpy().astype(self.dtype).reshape(self.N + (3,), order="F")
+            gx = self.orc.matvec(self.N, self.N, (1, 1, 1), self.fur, v, dtype=self.dtype)
+            chi = self.media[0 if self.nbatch == 1 else r]
+            out[r] = (v - chi[..., None] * gx).reshape(-1, order="F")
+        res = torch.from_numpy(out)
+        if y is None:
+            return res.reshape(x.shape)
+        y.copy_(res.reshape(y.shape))
+        return y
+
+    def restrict(self, idx):
+        if self.nbatch == 1:
+            return self
+        return OracleLS(self.orc, self.N, self.fur, [self.media[int(i)] for i in idx], self.dtype)
+
+
+def _oracle_residual(orc, N, fur, chi, x, b):
+    xa = x.reshape(N + (3,), order="F")
+    ax = xa - chi[..., None] * orc.matvec(N, N, (1, 1, 1), fur, xa)
+    return np.linalg.norm(ax.reshape(-1, order="F") - b) / np.linalg.norm(b)
+
+
+def test_gmres_ir_host_logic(gila, orc):
+    import torch
+    N = (6, 6, 6)
+    fur, _ = orc.self_operator_fur(N, model=0)
+    # a scaled Green function so that chi G is a sizeable perturbation of I
+    fur = fur * 40.0
+    chi = _sphere_medium(N, 2.0 + 0.5j)
+    A32 = OracleLS(orc, N, fur, [chi], np.complex64)
+    A64 = OracleLS(orc, N, fur, [chi], np.complex128)
+    b = torch.from_numpy(orc.complex_gaussian((2, A64.n), seed=21))
+    x, info = gila.solve.gmres_ir(A32, A64, b, tol=1e-11, restart=30, inner_tol=1e-4)
+    assert info.converged.all(), info
+    for r in range(2):
+        assert _oracle_residual(orc, N, fur, chi, x[r].numpy(), b[r].numpy()) <= 1e-11
+    # FP32 cycles stall near FP32 precision; the FP64 refinement carries on
+    assert info.cycles >= 2 and info.matvecs64 == sum(int((h > 1e-11).sum()) for h in info.history[:-1])
+
+
+def test_gmres_ir_per_rhs_media_and_subsets(gila, orc):
+    import torch
+    N = (4, 6, 4)
+    fur, _ = orc.self_operator_fur(N, model=0)
+    fur = fur * 40.0
+    media = [_sphere_medium(N, c) for c in (0.5 + 0.1j, 3.0 + 1.0j, 0j)]
+    A32 = OracleLS(orc, N, fur, media, np.complex64)
+    A64 = OracleLS(orc, N, fur, media, np.complex128)
+    b = torch.from_numpy(orc.complex_gaussian((3, A64.n), seed=22))
+    x, info = gila.solve.gmres_ir(A32, A64, b, tol=1e-10)
+    assert info.converged.all()
+    # chi = 0: A = I, so x = b after the first cycle
+    assert np.allclose(x[2].numpy(), b[2].numpy(), rtol=0, atol=1e-10 * np.abs(b[2].numpy()).max())
+    for r in range(3):
+        assert _oracle_residual(orc, N, fur, media[r], x[r].numpy(), b[r].numpy()) <= 1e-10
+        # alone and in the batch: same answer to the solve tolerance
+        xs, _ = gila.solve.gmres_ir(A32.restrict([r]), A64.restrict([r]), b[r:r + 1], tol=1e-10)
+        assert np.linalg.norm(xs[0].numpy() - x[r].numpy()) <= 1e-8 * np.linalg.norm(x[r].numpy())
+
+
+def test_gmres_zero_rhs_and_max_cycles(gila, orc):
+    import torch
+    N = (4, 4, 4)
+    fur, _ = orc.self_operator_fur(N, model=0)
+    chi = _sphere_medium(N, 1.0)
+    A32 = OracleLS(orc, N, fur * 40.0, [chi], np.complex64)
+    A64 = OracleLS(orc, N, fur * 40.0, [chi], np.complex128)
+    x, info = gila.solve.gmres_ir(A32, A64, torch.zeros((1, A64.n), dtype=torch.complex128))
+    assert info.converged.all() and info.cycles == 0 and not x.abs().max()
+    b = torch.from_numpy(orc.complex_gaussian((1, A64.n), seed=3))
+    x, info = gila.solve.gmres_ir(A32, A64, b, tol=1e-300, restart=2, max_cycles=3)
+    assert info.cycles == 3 and not info.converged.any() and len(info.history) == 4
+
+
+def _ls_setup(gila, orc, N, typ, model=0):
+    fur, _ = orc.self_operator_fur(N, model=model)
+    U = orc.dims(N, N, (1, 1, 1))[1]
+    vol = gila.GlaVol(N, ((1, 32),) * 3)
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), vol, egoFur=orc.fur_branches(fur.astype(typ), U), setTyp=typ)
+    return gila.GlaOpr(mem), fur
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ,tol", [(np.complex128, 1e-12), (np.complex64, 1e-5)])
+def test_ls_apply_matches_oracle(gila, orc, gpu, typ, tol):
+    import torch
+    N = (16, 12, 8)
+    print(f"case ls_apply N={N} {np.dtype(typ).name}", flush=True)
+    G, fur = _ls_setup(gila, orc, N, typ)
+    media = [_sphere_medium(N, 2.0 + 0.5j), _sphere_medium(N, -1.0 + 3.0j, 0.3)]
+    x = orc.complex_gaussian((2,) + N + (3,), seed=31)
+    xd = torch.from_numpy(np.stack([x[r].reshape(-1, order="F") for r in range(2)]).astype(typ)).cuda()
+    shared = gila.solve.LippmannSchwinger(G, media[0].astype(typ))
+    batched = gila.solve.LippmannSchwinger(G, np.stack(media, axis=3).astype(typ))
+    ys = shared.apply(xd).cpu().numpy()
+    yb = batched.apply(xd).cpu().numpy()
+    for r in range(2):
+        gx = orc.matvec(N, N, (1, 1, 1), fur, x[r])
+        for y, chi in ((ys[r], media[0]), (yb[r], media[r])):
+            ref = x[r] - chi[..., None] * gx
+            assert orc.rel_l2(y.reshape(N + (3,), order="F"), ref) <= tol
+    with pytest.raises(gila.DimensionMismatch):
+        gila.solve.LippmannSchwinger(G, media[0][:, :, :4].astype(typ))
+    with pytest.raises(gila.ArgumentError):
+        gila.solve.LippmannSchwinger(G, media[0].astype(np.complex128 if typ == np.complex64 else np.complex64))
+    with pytest.raises(gila.DimensionMismatch):
+        batched.apply(xd[:1])
+
+
+@pytest.mark.gpu
+def test_gmres_ir_on_gpu(gila, orc, gpu):
+    import torch
+    N = (16, 16, 16)
+    print(f"case gmres_ir N={N}", flush=True)
+    G32, fur = _ls_setup(gila, orc, N, np.complex64, model=2)
+    G64, _ = _ls_setup(gila, orc, N, np.complex128, model=2)
+    media = [_sphere_medium(N, c) for c in (3.0 + 0.2j, 10.0 + 1.0j)]
+    m = np.stack(media, axis=3)
+    A32 = gila.solve.LippmannSchwinger(G32, m.astype(np.complex64))
+    A64 = gila.solve.LippmannSchwinger(G64, m)
+    b = torch.from_numpy(orc.complex_gaussian((2, A64.n), seed=41)).cuda()
+    x, info = gila.solve.gmres_ir(A32, A64, b, tol=1e-10)
+    assert info.converged.all(), info
+    for r in range(2):
+        assert _oracle_residual(orc, N, fur, media[r], x[r].cpu().numpy(), b[r].cpu().numpy()) <= 1e-9
