@@ -41,7 +41,36 @@ def parse():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--counters", choices=["c64", "c128"], default=None,
+                    help="short program for rocprofv3 counter passes: a 512 MiB device copy (the calibration "
+                         "of FETCH_SIZE/WRITE_SIZE) and three matvecs of the given type; prints no JSON")
     return ap.parse_args()
+
+
+def counters_program(a):
+    """A fixed, short sequence of dispatches for --pmc runs: one copy of known
+    size (512 MiB read, 512 MiB written), then three matvecs."""
+    import torch
+    import __graft_entry__ as ge
+    gila = ge.load_package()
+    from gila_amd import inputs
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    typ = np.complex64 if a.counters == "c64" else np.complex128
+    N = (a.cells,) * 3
+    mem = make_operator(gila, N, typ)
+    x = inputs.complex_gaussian(N + (3,), seed=2).astype(typ)
+    xd = torch.from_numpy(x.ravel(order="F").copy()).cuda()
+    yd = torch.empty_like(xd)
+    from gila_amd import _capi
+    st = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):
+        _capi.check(gila.lib().gla_opr_apply(mem._h, ctypes.c_void_p(xd.data_ptr()), xd.numel(),
+                                             ctypes.c_void_p(yd.data_ptr()), yd.numel(), 1, ctypes.c_void_p(st)))
+    torch.cuda.synchronize()
+    print("counters program done", flush=True)
 
 
 def make_operator(gila, N, typ):
@@ -94,6 +123,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); multi-GPU runs are not part of this "
               "build", file=sys.stderr)
         sys.exit(2)
+    if a.counters:
+        counters_program(a)
+        return
     import torch
     import __graft_entry__ as ge
     gila = ge.load_package()

@@ -21,4 +21,4 @@ from .gla import (  # noqa: F401
     glaSze, glaVolEveGen, isadjoint, isdiag, isexternaloperator, ishermitian, isposdef, isselfoperator,
     issymmetric, mul_, show, size,
 )
-from . import solve  # noqa: F401,E402
+from . import serve, solve  # noqa: F401,E402

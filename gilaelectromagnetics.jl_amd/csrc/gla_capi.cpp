@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "gla_krylov.hpp"
 #include "gla_launch.hpp"
 #include "gla_setup.hpp"
 
@@ -578,6 +579,55 @@ int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs)
 {
     if (!h) return -1;
     return (h->geom.w1s + h->geom.w2s) * nrhs * (int64_t)elem_bytes(h->dims.dtype);
+}
+
+int64_t gla_krylov_dots_workspace(int64_t k, int64_t nv, int64_t n)
+{
+    if (k < 0 || nv < 0 || n < 0) return -1;
+    const int64_t chunk = gla::krylov_chunk(n);
+    return k * nv * ((n + chunk - 1) / chunk) * 16;
+}
+
+static gla_status krylov_check(int32_t dtype, int64_t k, int64_t nv, int64_t n)
+{
+    if (dtype != GLA_C64 && dtype != GLA_C128) return fail(GLA_ERR_TYPE, "dtype must be GLA_C64 or GLA_C128");
+    if (k < 0 || nv < 0 || n < 0) return fail(GLA_ERR_ARG, "negative size");
+    if (k > 65535) return fail(GLA_ERR_ARG, "at most 65535 right-hand sides per call");
+    if (nv > (1 << 20)) return fail(GLA_ERR_ARG, "too many basis vectors");
+    return GLA_OK;
+}
+
+gla_status gla_krylov_dots(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
+                           int64_t v_vec_stride, const void* w, int64_t w_rhs_stride, void* part,
+                           int64_t part_bytes, void* h, void* stream)
+{
+    if (gla_status s = krylov_check(dtype, k, nv, n)) return s;
+    if (k == 0 || nv == 0) return GLA_OK;
+    if (!V || !w || !h || (!part && n > 0)) return fail(GLA_ERR_ARG, "null argument");
+    if (part_bytes < gla_krylov_dots_workspace(k, nv, n))
+        return fail(GLA_ERR_SIZE, "partial-sum workspace smaller than gla_krylov_dots_workspace()");
+    if (n == 0) return fail(GLA_ERR_ARG, "empty vectors");
+    const hipStream_t st = (hipStream_t)stream;
+    const hipError_t e = dtype == GLA_C64
+        ? gla::run_krylov_dots<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, st)
+        : gla::run_krylov_dots<double>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, st);
+    return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
+}
+
+gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
+                           int64_t v_vec_stride, const void* h, int64_t h_rhs_stride, void* w,
+                           int64_t w_rhs_stride, double s, void* stream)
+{
+    if (gla_status st = krylov_check(dtype, k, nv, n)) return st;
+    if (k == 0 || n == 0) return GLA_OK;
+    if (!w || (nv > 0 && (!V || !h))) return fail(GLA_ERR_ARG, "null argument");
+    const hipStream_t hs = (hipStream_t)stream;
+    const hipError_t e = dtype == GLA_C64
+        ? gla::run_krylov_axpy<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, h, h_rhs_stride, w,
+                                      w_rhs_stride, s, hs)
+        : gla::run_krylov_axpy<double>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, h, h_rhs_stride, w,
+                                       w_rhs_stride, s, hs);
+    return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
 }
 
 }  // extern "C"

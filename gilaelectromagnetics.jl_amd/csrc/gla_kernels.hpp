@@ -42,6 +42,34 @@ __device__ __forceinline__ void axis_lookup(const AxisMap& a, int p, int k, int&
     }
 }
 
+// Symmetric 3x3 multiply of one branch column (mulKerHst!,
+// src/glaActSup.jl:879-899): w[c][e] at axis-3 frequency k3 = t + TP e.
+// gb points at the column's entry of the branch in the device layout
+// gf[((b U3 + j3) U2 U1 + j2 U1 + j1) * 6 + comp], uu = U1 U2, so each point
+// adds j3 * uu records; s1, s2 are the column's mirror signs.
+template <typename R, int E, int TP>
+__device__ __forceinline__ void g_multiply(cpx<R> (&w)[3][E], const cpx<R>* __restrict__ gb, int uu,
+                                           const AxisMap& a3, int p3, int t, int s1, int s2, bool act)
+{
+    const R s12 = R(s1 * s2);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int k3 = t + TP * e;
+        int j3, s3;
+        axis_lookup(a3, p3, k3, j3, s3);
+        if (act) {
+            const cpx<R>* g = gb + (int64_t)(j3 * uu) * 6;
+            const cpx<R> gxx = g[0], gyy = g[1], gzz = g[2];
+            const R s13 = R(s1 * s3), s23 = R(s2 * s3);
+            const cpx<R> gxy = s12 * g[3], gxz = s13 * g[4], gyz = s23 * g[5];
+            const cpx<R> x0 = w[0][e], x1 = w[1][e], x2 = w[2][e];
+            w[0][e] = gxx * x0 + gxy * x1 + gxz * x2;
+            w[1][e] = gxy * x0 + gyy * x1 + gyz * x2;
+            w[2][e] = gxz * x0 + gyz * x1 + gzz * x2;
+        }
+    }
+}
+
 // --------------------------------------------------------------------------
 // Pass 1: x-forward on contiguous rows.  One pencil = one (i2,i3,c) row of X.
 // --------------------------------------------------------------------------
@@ -297,31 +325,12 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
     int j1, s1, j2, s2;
     axis_lookup(a1, p1, act ? k1 : 0, j1, s1);
     axis_lookup(a2, p2, k2, j2, s2);
-    const R s12 = R(s1 * s2);
 
-    // symmetric 3x3 multiply of branch (p1,p2,p3) (mulKerHst!, src/glaActSup.jl:879-899)
+    const int uu = a1.U * a2.U;
+    const int gcol = j2 * a1.U + j1;
     auto mul_g = [&](cpx<R>(&w)[3][E], int p3) {
         const int b = (p1 << 2) | (p2 << 1) | p3;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            // G^ loads in groups of GG points: bounds the registers the
-            // scheduler spends on loads in flight
-            if constexpr (GG < E)
-                if (e > 0 && e % GG == 0) __builtin_amdgcn_sched_barrier(0);
-            const int k3 = t + TP * e;
-            int j3, s3;
-            axis_lookup(a3, p3, k3, j3, s3);
-            if (act) {
-                const cpx<R>* g = gf + ((((int64_t)b * a3.U + j3) * a2.U + j2) * a1.U + j1) * 6;
-                const cpx<R> gxx = g[0], gyy = g[1], gzz = g[2];
-                const R s13 = R(s1 * s3), s23 = R(s2 * s3);
-                const cpx<R> gxy = s12 * g[3], gxz = s13 * g[4], gyz = s23 * g[5];
-                const cpx<R> x0 = w[0][e], x1 = w[1][e], x2 = w[2][e];
-                w[0][e] = gxx * x0 + gxy * x1 + gxz * x2;
-                w[1][e] = gxy * x0 + gyy * x1 + gyz * x2;
-                w[2][e] = gxz * x0 + gyz * x1 + gzz * x2;
-            }
-        }
+        g_multiply<R, E, TP>(w, gf + ((int64_t)b * a3.U * uu + gcol) * 6, uu, a3, p3, t, s1, s2, act);
     };
     auto fwd = [&](cpx<R>(&w)[3][E]) {
         if constexpr (KB == 3) {
@@ -428,6 +437,143 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
             if constexpr (MODE == 2)
                 if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = accb[c][e];
         }
+}
+
+// --------------------------------------------------------------------------
+// Pass 3, software-pipelined: a persistent form of k_zpass for S3 <= B3.
+// Each workgroup walks its XCD's part of the tile order (tiles l = xcd *
+// per_xcd + q + i * bpx for its slot q of bpx per XCD) and loads the next
+// tile's columns while the current one is transformed, so the memory phase of
+// one tile overlaps the arithmetic of the other.  The next tile's loads are
+// issued after the even parity's G^ multiply: vector-memory completions are
+// counted in order, so issuing them earlier would make the even parity's G^
+// loads wait for them.  Same arithmetic as k_zpass with both parities split
+// first.
+// --------------------------------------------------------------------------
+template <typename R, int N, int TP, int PX, int MODE, int KB, int PF>
+__global__ __launch_bounds__(TP * PX, 2) void k_zpipe(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int T3,
+                                                   int Z, int64_t w2s, const cpx<R>* __restrict__ gf,
+                                                   AxisMap a1, AxisMap a2, AxisMap a3,
+                                                   const cpx<R>* __restrict__ phz,
+                                                   const cpx<R>* __restrict__ tw,
+                                                   const int* __restrict__ order, int per_xcd)
+{
+    static_assert(MODE != 1, "the pipelined pass needs S3 <= B3");
+    constexpr int E = N / TP;
+    __shared__ cpx<R> lds[KB * PX * N];
+    int xl = threadIdx.x % PX;
+    int t = threadIdx.x / PX;
+    const int xcd = blockIdx.x % 8, bpx = gridDim.x / 8;
+    int il = blockIdx.x / 8;
+    if (il >= per_xcd) return;   // whole workgroup
+    const int64_t cs = (int64_t)Z * L2 * L1;
+    const int64_t zs = (int64_t)L2 * L1;
+    cpx<R>* const base = W2 + (int64_t)blockIdx.z * w2s;
+    const int uu = a1.U * a2.U;
+
+    auto fwd = [&](cpx<R>(&w)[3][E]) {
+        if constexpr (KB == 3) {
+            PencilFft<R, N, TP, PX, 1, false, 0, 3, PX * N>::run(w, lds, t, xl, tw);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, false>::run(w[c], lds, t, xl, tw);
+        }
+    };
+    auto inv = [&](cpx<R>(&w)[3][E]) {
+        if constexpr (KB == 3) {
+            PencilFft<R, N, TP, PX, 1, true, 0, 3, PX * N>::run(w, lds, t, xl, tw);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, true>::run(w[c], lds, t, xl, tw);
+        }
+    };
+    // this thread's column of a tile (order entry tile | m2 << 16, -1 padding)
+    auto column = [&](int tile, int& m1, int& m2) {
+        m1 = (tile & 0xffff) * PX + xl;
+        m2 = tile >> 16;
+        return tile >= 0 && m1 < L1;
+    };
+    // components [c0, c1) of a tile's columns
+    auto load = [&](int tile, cpx<R>(&xa)[3][E], int c0, int c1) {
+        int m1, m2;
+        const bool ok = column(tile, m1, m2);
+        const cpx<R>* col = base + (ok ? (int64_t)m2 * L1 + m1 : 0);
+#pragma unroll
+        for (int c = c0; c < c1; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int n = t + TP * e;
+                xa[c][e] = (ok && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
+            }
+    };
+
+    int tile = order[xcd * per_xcd + il];
+    cpx<R> xa[3][E];
+    load(tile, xa, 0, 3);
+    for (;;) {
+        // table pointers opaque per tile: keeps the twiddle and phase loads
+        // inside the loop instead of holding their values in registers
+        // across all tiles
+        // the thread index opaque per tile: what derives from it (offsets,
+        // phases, mirror lookups) is recomputed per tile rather than held in
+        // registers across the loop
+        asm volatile("" : "+v"(t), "+v"(xl));
+        int m1, m2;
+        const bool act = column(tile, m1, m2);
+        const int p1 = (act && m1 >= a1.B) ? 1 : 0, k1 = act ? m1 - p1 * a1.B : 0;
+        const int p2 = (act && m2 >= a2.B) ? 1 : 0, k2 = act ? m2 - p2 * a2.B : 0;
+        int j1, s1, j2, s2;
+        axis_lookup(a1, p1, k1, j1, s1);
+        axis_lookup(a2, p2, k2, j2, s2);
+        const cpx<R>* gc = gf + (int64_t)(j2 * a1.U + j1) * 6;
+        cpx<R> wv[2][3][E];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int n = t + TP * e;
+                wv[0][c][e] = xa[c][e];
+                wv[1][c][e] = phz[n] * xa[c][e];
+            }
+        const int nil = il + bpx;
+        const bool more = nil < per_xcd;   // the same for the whole workgroup
+        const int ntile = more ? order[xcd * per_xcd + nil] : -1;
+        const int b0 = (p1 << 2) | (p2 << 1);
+        fwd(wv[0]);
+        g_multiply<R, E, TP>(wv[0], gc + (int64_t)b0 * a3.U * uu * 6, uu, a3, 0, t, s1, s2, act);
+        // the next tile's first PF components load during the rest of this
+        // tile's transforms, the others during the odd parity's inverse
+        __builtin_amdgcn_sched_barrier(0);
+        if (more && PF > 0) load(ntile, xa, 0, PF);
+        __builtin_amdgcn_sched_barrier(0);
+        inv(wv[0]);
+        fwd(wv[1]);
+        g_multiply<R, E, TP>(wv[1], gc + (int64_t)(b0 | 1) * a3.U * uu * 6, uu, a3, 1, t, s1, s2, act);
+        if constexpr (PF >= 0 && PF < 3) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (more) load(ntile, xa, PF, 3);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        inv(wv[1]);
+        if (act) {
+            cpx<R>* col = base + (int64_t)m2 * L1 + m1;
+            const int top = T3 < N ? T3 : N;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int n = t + TP * e;
+                    const cpx<R> po = cmulc(wv[1][c][e], phz[n]);
+                    if (n < top) col[c * cs + (int64_t)n * zs] = wv[0][c][e] + po;
+                    if constexpr (MODE == 2)
+                        if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = wv[0][c][e] - po;
+                }
+        }
+        if (!more) break;
+        il = nil;
+        tile = ntile;
+        if constexpr (PF < 0) load(tile, xa, 0, 3);
+    }
 }
 
 // --------------------------------------------------------------------------

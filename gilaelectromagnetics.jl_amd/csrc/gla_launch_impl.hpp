@@ -36,6 +36,19 @@
 #ifndef GLA_ZPADSH
 #define GLA_ZPADSH 3
 #endif
+// pass 3 as a persistent, software-pipelined kernel when S3 <= B3 (off: at
+// 128^3 it measured 230-240 us against 211 us for k_zpass, the persistent
+// loop costing more registers than the overlap gains)
+#ifndef GLA_ZPIPE
+#define GLA_ZPIPE 0
+#endif
+// components of the next tile loaded early in the pipelined pass 3
+#ifndef GLA_ZPF_F
+#define GLA_ZPF_F 2
+#endif
+#ifndef GLA_ZPF_D
+#define GLA_ZPF_D 2
+#endif
 
 namespace gla {
 
@@ -150,13 +163,33 @@ static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nr
     using K = LenK<R, N>;
     constexpr int PX = Cfg<R>::PXZ;
     constexpr int SC = Cfg<R>::ZSCHED;
+    constexpr int PF = sizeof(R) == 8 ? GLA_ZPF_D : GLA_ZPF_F;
     dim3 grid(8 * tb.zper_xcd, 1, nrhs);
     dim3 blk(K::TP_Z * PX);
     const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);
 #define GLA_ZARGS                                                                                   \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
         tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd
-    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    if (GLA_ZPIPE && mode != 1 && !K::WL_Z) {
+        // one resident wave of workgroups: blocks per CU from the occupancy
+        // of this kernel, 8 XCDs of CUs/8 each
+        static int bpx = 0;
+        if (bpx == 0) {
+            int dev = 0, ncu = 0, nb = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                return hipErrorInvalidValue;
+            const hipError_t oe = mode == 0
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zpipe<R, N, K::TP_Z, PX, 0, K::KB_Z, PF>, K::TP_Z * PX, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zpipe<R, N, K::TP_Z, PX, 2, K::KB_Z, PF>, K::TP_Z * PX, 0);
+            if (oe != hipSuccess) return oe;
+            bpx = (nb < 1 ? 1 : nb) * ((ncu + 7) / 8);
+        }
+        const int nb_xcd = bpx < tb.zper_xcd ? bpx : tb.zper_xcd;
+        dim3 pgrid(8 * nb_xcd, 1, nrhs);
+        if (mode == 0) hipLaunchKernelGGL((k_zpipe<R, N, K::TP_Z, PX, 0, K::KB_Z, PF>), pgrid, blk, 0, st, GLA_ZARGS);
+        else hipLaunchKernelGGL((k_zpipe<R, N, K::TP_Z, PX, 2, K::KB_Z, PF>), pgrid, blk, 0, st, GLA_ZARGS);
+    } else if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
     else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
     else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
 #undef GLA_ZARGS

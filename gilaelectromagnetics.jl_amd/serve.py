@@ -1,0 +1,624 @@
+"""Continuous-batching server for scattering solves (I - chi G) E = E_inc.
+
+The reference stops at the operator; its examples solve the
+Lippmann-Schwinger system with IterativeSolvers (LippmannSchwinger,
+examples/GilaOperators.jl:57-96; examples/scattering_guide.jl:65-68) and
+cache Green data per (type, cells, scale) in preload files
+(examples/GilaOperators.jl:28-55).  Here requests of different domains
+arrive over time and are solved together on one GPU:
+
+* Operators: the FP32 and FP64 Fourier Green data of every grid in use stay
+  resident, keyed by (cells, scale, frqPhz), and are reused by later
+  requests on the same grid.
+* Slots: each grid owns a slab of request slots -- an FP32 Krylov basis
+  (restart + 1 vectors), the FP64 iterate, right-hand side and residual, and
+  the medium -- allocated once and reused.  Requests are admitted in arrival
+  order when a slot is free or a new slab fits the memory budget; the budget
+  counts every byte of a slab and of the operators (Green data and the
+  library's per-RHS work buffers).
+* Rounds: every round runs one Arnoldi step for all active requests of a grid
+  in one batched Lippmann-Schwinger apply (gla_ls_apply, one medium per
+  right-hand side), with the grids on separate streams so their kernels
+  overlap, and one host synchronisation for all grids.  A request joins at
+  any round: the Gram-Schmidt products over a slab use the columns up to the
+  largest step in the batch, and a request's columns beyond its own step are
+  zero.
+* Precision: GMRES(restart) cycles in FP32 on the residual equation; at the
+  end of each cycle the update and the residual b - A x are formed in FP64
+  (iterative refinement) and the request completes when
+  ||b - A x|| / ||b|| <= tol by that FP64 residual.
+
+A result does not depend on what else is in the batch beyond rounding: the
+matvec of each right-hand side is computed by the same kernels whatever the
+batch, and the Gram-Schmidt, rotations and updates of a request use only its
+own slot.
+"""
+import contextlib
+import ctypes
+import time
+from fractions import Fraction
+
+import numpy as np
+
+from . import _capi as capi
+from .solve import DGKS, KrylovOps, combine, gs_pass
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class SolveRequest:
+    """One scattering solve: chi (c1,c2,c3) and E_inc (c1,c2,c3,3), complex,
+    column-major, on a grid of `cells` cubic cells of side `scale`."""
+
+    _next_id = 0
+
+    def __init__(self, cells, chi, e_inc, tol=1e-6, scale=Fraction(1, 32), frq=1.0 + 0j, rid=None):
+        self.cells = tuple(int(c) for c in cells)
+        chi = np.asarray(chi)
+        e_inc = np.asarray(e_inc)
+        if chi.shape != self.cells or e_inc.shape != self.cells + (3,):
+            raise ValueError("chi must be (c1,c2,c3) and e_inc (c1,c2,c3,3)")
+        self.chi = chi
+        self.e_inc = e_inc
+        self.tol = float(tol)
+        self.scale = Fraction(scale)
+        self.frq = complex(frq)
+        if rid is None:
+            rid = SolveRequest._next_id
+            SolveRequest._next_id += 1
+        self.rid = rid
+        self.t_arrive = None
+        self.t_admit = None
+        self.t_done = None
+        self.result = None          # FP64 field (c1,c2,c3,3), numpy
+        self.relres = None
+        self.converged = False
+        self.cycles = 0
+        self.steps = 0
+
+    @property
+    def key(self):
+        return (self.cells, self.scale, self.frq)
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_arrive
+
+
+class DeviceOps:
+    """FP32 and FP64 Lippmann-Schwinger applies of one grid through libgila_amd."""
+
+    def __init__(self, cells, scale, frq, device=0):
+        from .gla import GlaKerOpt, GlaOprMem, GlaVol
+        vol = GlaVol(cells, (scale,) * 3)
+        self.cells = tuple(cells)
+        self.ncell = int(np.prod(cells))
+        self.n = 3 * self.ncell
+        self.mem32 = GlaOprMem(GlaKerOpt(frq, devMod=True, device=device), vol, setTyp=np.complex64)
+        self.mem64 = GlaOprMem(GlaKerOpt(frq, devMod=True, device=device), vol, setTyp=np.complex128)
+        L = capi.lib()
+        self._ws32 = L.gla_opr_workspace_bytes(self.mem32._h, 1)
+        self._ws64 = L.gla_opr_workspace_bytes(self.mem64._h, 1)
+        # device G^ bytes of both operators: 8 branches x U1 U2 U3 x 6 components
+        U = [int(u) for u in self.mem64._dims.fur_ext]
+        self.fixed_bytes = 8 * 6 * int(np.prod(U)) * (8 + 16)
+
+    def ws_bytes(self, nrhs32, nrhs64):
+        return self._ws32 * nrhs32 + self._ws64 * nrhs64
+
+    def _apply(self, mem, x, chi, y, stream):
+        k = x.shape[0]
+        capi.check(capi.lib().gla_ls_apply(mem._h, ctypes.c_void_p(chi.data_ptr()), chi.numel(),
+                                           self.ncell, ctypes.c_void_p(x.data_ptr()), x.numel(),
+                                           ctypes.c_void_p(y.data_ptr()), y.numel(), k,
+                                           ctypes.c_void_p(stream)))
+
+    def apply32(self, x, chi, y, stream):
+        self._apply(self.mem32, x, chi, y, stream)
+
+    def apply64(self, x, chi, y, stream):
+        self._apply(self.mem64, x, chi, y, stream)
+
+
+class _Grid:
+    """Slab of request slots on one grid; slots 0..k-1 are active."""
+
+    def __init__(self, ops, cap, restart, inner, device, gs="cgs2"):
+        torch = _torch()
+        self.ops = ops
+        self.cap = cap
+        self.m = restart
+        self.n = ops.n
+        self.dev = device
+        self.inner = inner
+        self.gs = gs
+        cdt = torch.complex64 if inner == "fp32" else torch.complex128
+        self.cdt = cdt
+        n, m = self.n, restart
+        self.V = torch.zeros((cap, m + 1, n), dtype=cdt, device=device)
+        self.W = torch.empty((cap, n), dtype=cdt, device=device)
+        self.Xin = torch.empty((cap, n), dtype=cdt, device=device)
+        self.chi_in = torch.zeros((cap, ops.ncell), dtype=cdt, device=device)
+        self.chi64 = torch.zeros((cap, ops.ncell), dtype=torch.complex128, device=device)
+        self.x = torch.zeros((cap, n), dtype=torch.complex128, device=device)
+        self.b = torch.zeros((cap, n), dtype=torch.complex128, device=device)
+        self.r = torch.zeros((cap, n), dtype=torch.complex128, device=device)
+        self.t64 = torch.empty((cap, n), dtype=torch.complex128, device=device)
+        self.H = np.zeros((cap, m + 1, m), np.complex128)
+        self.cs = np.zeros((cap, m))
+        self.sn = np.zeros((cap, m), np.complex128)
+        self.g = np.zeros((cap, m + 1), np.complex128)
+        self.j = np.zeros(cap, np.int64)
+        self.rn = np.zeros(cap)
+        self.bn = np.ones(cap)
+        self.reqs = []
+        self.stream = torch.cuda.Stream(device=device) if str(device).startswith("cuda") else None
+        self.kry = KrylovOps()
+        self.last_used = time.perf_counter()
+        self.pending = None
+        self.reorth = None
+
+    @staticmethod
+    def slot_bytes(n, ncell, restart, inner):
+        s = 8 if inner == "fp32" else 16
+        return (restart + 3) * n * s + ncell * (s + 16) + 4 * n * 16
+
+    @property
+    def k(self):
+        return len(self.reqs)
+
+    def ctx(self):
+        torch = _torch()
+        return torch.cuda.stream(self.stream) if self.stream is not None else contextlib.nullcontext()
+
+    def stream_handle(self):
+        torch = _torch()
+        if self.stream is not None:
+            return self.stream.cuda_stream
+        return torch.cuda.current_stream().cuda_stream if torch.cuda.is_available() else 0
+
+    def apply_inner(self, x, chi, y):
+        (self.ops.apply32 if self.inner == "fp32" else self.ops.apply64)(x, chi, y, self.stream_handle())
+
+    def apply64(self, x, chi, y):
+        self.ops.apply64(x, chi, y, self.stream_handle())
+
+    # --- slot management ---------------------------------------------------
+    def start_cycle(self, s):
+        """Arnoldi restart of slot s from its residual r (norm rn[s])."""
+        self.V[s, 1:].zero_()
+        self.V[s, 0].copy_((self.r[s] / self.rn[s]).to(self.cdt))
+        self.H[s] = 0
+        self.g[s] = 0
+        self.g[s, 0] = 1.0
+        self.j[s] = 0
+
+    def admit(self, req, now):
+        torch = _torch()
+        s = self.k
+        self.reqs.append(req)
+        req.t_admit = now
+        chi = torch.from_numpy(np.ascontiguousarray(req.chi.reshape(-1, order="F")).astype(np.complex128))
+        b = torch.from_numpy(np.ascontiguousarray(req.e_inc.reshape(-1, order="F")).astype(np.complex128))
+        bn = float(np.linalg.norm(req.e_inc))
+        with self.ctx():
+            self.chi64[s].copy_(chi.to(self.dev, non_blocking=False))
+            self.chi_in[s].copy_(self.chi64[s].to(self.cdt))
+            self.b[s].copy_(b.to(self.dev, non_blocking=False))
+            self.x[s].zero_()
+            self.r[s].copy_(self.b[s])
+        self.bn[s] = bn if bn > 0 else 1.0
+        self.rn[s] = bn
+        if bn == 0:
+            return False
+        with self.ctx():
+            self.start_cycle(s)
+        return True
+
+    def remove(self, s):
+        """Drop slot s, moving the last active slot into it."""
+        last = self.k - 1
+        if s != last:
+            with self.ctx():
+                for t in (self.V, self.chi_in, self.chi64, self.x, self.b, self.r):
+                    t[s].copy_(t[last])
+            for a in (self.H, self.cs, self.sn, self.g, self.j, self.rn, self.bn):
+                a[s] = a[last]
+            self.reqs[s] = self.reqs[last]
+        self.reqs.pop()
+
+    # --- one Arnoldi step for all active slots -------------------------------
+    def launch_step(self):
+        """Gather each slot's current basis vector, apply A to the batch and
+        run one classical Gram-Schmidt pass over the slab's columns 0..jm
+        (zero beyond each slot's own step); the coefficients and the norms
+        before and after the pass are copied to the host."""
+        torch = _torch()
+        k = self.k
+        jm = int(self.j[:k].max())
+        with self.ctx():
+            jt = torch.from_numpy(self.j[:k].copy()).to(self.dev)
+            ar = torch.arange(k, device=self.dev)
+            self.Xin[:k] = self.V[ar, jt]
+            self.apply_inner(self.Xin[:k], self.chi_in[:k], self.W[:k])
+            n0 = torch.linalg.vector_norm(self.W[:k], dim=1)
+            h = gs_pass(self.kry, self.V, k, jm + 1, self.W, self.stream_handle())
+            if self.gs == "cgs2":
+                h = h + gs_pass(self.kry, self.V, k, jm + 1, self.W, self.stream_handle())
+            n1 = torch.linalg.vector_norm(self.W[:k], dim=1)
+            host = [t.to("cpu", non_blocking=True) for t in (h, n0.double(), n1.double())]
+            self.pending = (host, jt, ar, jm)
+
+    def launch_reorth(self):
+        """The second Gram-Schmidt pass: always ("cgs2"), never ("cgs1"), or
+        when a slot lost more than 1 - 1/sqrt(2) of its norm ("dgks", which
+        costs a host round trip).  Returns whether one was launched."""
+        torch = _torch()
+        (h, n0, n1), jt, ar, jm = self.pending
+        self.reorth = None
+        if self.gs != "dgks" or (self.gs == "dgks" and not (n1.numpy() < DGKS * n0.numpy()).any()):
+            return False
+        with self.ctx():
+            h2 = gs_pass(self.kry, self.V, self.k, jm + 1, self.W, self.stream_handle())
+            n2 = torch.linalg.vector_norm(self.W[:self.k], dim=1).double()
+            self.reorth = (h2.to("cpu", non_blocking=True), n2.to("cpu", non_blocking=True))
+        return True
+
+    def finish_step(self):
+        """Host part of the step: the next basis vector, Givens rotations and
+        residual estimates.  Returns the slots whose cycle ends."""
+        torch = _torch()
+        (h, n0, n1), jt, ar, jm = self.pending
+        self.pending = None
+        k = self.k
+        m = self.m
+        hcol = h.numpy()
+        hn = n1.numpy()
+        if self.reorth is not None:
+            h2, n2 = self.reorth
+            self.reorth = None
+            hcol = hcol + h2.numpy()
+            hn = n2.numpy()
+        with self.ctx():
+            scale = torch.from_numpy(np.maximum(hn, 1e-30)).to(self.dev, self.W.real.dtype)
+            self.V[ar, jt + 1] = self.W[:k] / scale.unsqueeze(1)
+        js = self.j[:k].copy()
+        if k <= 4:
+            self._rotate_scalar(hcol, hn, js, jm)
+        else:
+            self._rotate_vector(hcol, hn, js, jm)
+        ar_h = np.arange(k)
+        res = np.abs(self.g[ar_h, js + 1])
+        est = res * self.rn[:k] / self.bn[:k]
+        self.j[:k] = js + 1
+        ended = []
+        for s in range(k):
+            req = self.reqs[s]
+            req.steps += 1
+            if js[s] + 1 == m or res[s] <= self.inner_tol or est[s] <= 0.5 * req.tol:
+                ended.append(s)
+        return ended
+
+    def _rotate_scalar(self, hcol, hn, js, jm):
+        """Givens update of each slot's Hessenberg column with Python complex
+        scalars (small batches)."""
+        for s in range(len(js)):
+            j = int(js[s])
+            col = [complex(v) for v in hcol[s, :j + 1]] + [complex(hn[s])]
+            cs, sn = self.cs[s], self.sn[s]
+            for i in range(j):
+                c, z = float(cs[i]), complex(sn[i])
+                a, b = col[i], col[i + 1]
+                col[i] = c * a + z * b
+                col[i + 1] = -z.conjugate() * a + c * b
+            a, b = col[j], col[j + 1]
+            den = (abs(a) ** 2 + abs(b) ** 2) ** 0.5
+            if den == 0:
+                c, z = 1.0, 0j
+            elif abs(a) == 0:
+                c, z = 0.0, 1 + 0j
+            else:
+                c, z = abs(a) / den, (a / abs(a)) * b.conjugate() / den
+            cs[j], sn[j] = c, z
+            col[j] = c * a + z * b
+            col[j + 1] = 0j
+            self.H[s, :j + 2, j] = col
+            gj = complex(self.g[s, j])
+            self.g[s, j + 1] = -z.conjugate() * gj
+            self.g[s, j] = c * gj
+
+    def _rotate_vector(self, hcol, hn, js, jm):
+        """The same update for all slots at once (numpy over the batch)."""
+        k = len(js)
+        ar_h = np.arange(k)
+        cols = np.zeros((k, self.m + 1), np.complex128)
+        cols[:, :jm + 1] = np.where(np.arange(jm + 1)[None, :] <= js[:, None], hcol[:, :jm + 1], 0)
+        cols[ar_h, js + 1] = hn
+        for i in range(jm):
+            act = i < js
+            c, sn = self.cs[:k, i], self.sn[:k, i]
+            a, b = cols[:, i].copy(), cols[:, i + 1].copy()
+            cols[:, i] = np.where(act, c * a + sn * b, a)
+            cols[:, i + 1] = np.where(act, -np.conj(sn) * a + c * b, b)
+        a, b = cols[ar_h, js], cols[ar_h, js + 1]
+        den = np.hypot(np.abs(a), np.abs(b))
+        aa = np.abs(a)
+        safe = np.where(aa > 0, aa, 1.0)
+        dd = np.where(den > 0, den, 1.0)
+        c = np.where(den == 0, 1.0, np.where(aa == 0, 0.0, aa / dd))
+        sn = np.where(den == 0, 0.0, np.where(aa == 0, 1.0, (a / safe) * np.conj(b) / dd))
+        self.cs[ar_h, js] = c
+        self.sn[ar_h, js] = sn
+        cols[ar_h, js] = c * a + sn * b
+        cols[ar_h, js + 1] = 0.0
+        self.H[ar_h, :, js] = cols
+        gj = self.g[ar_h, js].copy()
+        self.g[ar_h, js + 1] = -np.conj(sn) * gj
+        self.g[ar_h, js] = c * gj
+
+    def launch_refine(self, slots):
+        """FP64 update x += V y |r| and residual r = b - A x for the slots whose
+        cycle ended; residual norms go to the host."""
+        torch = _torch()
+        with self.ctx():
+            for s in slots:
+                kk = int(self.j[s])
+                R = self.H[s, :kk, :kk]
+                y = np.linalg.solve(np.triu(R), self.g[s, :kk]) if kk else np.zeros(0, np.complex128)
+                d = self.Xin[s]
+                d.zero_()
+                if kk:
+                    combine(self.kry, self.V, s, kk, torch.from_numpy(y).to(self.dev), d, self.stream_handle())
+                self.x[s] += d.to(torch.complex128) * self.rn[s]
+                self.reqs[s].cycles += 1
+            idx = torch.tensor(slots, device=self.dev)
+            nb = len(slots)
+            xin = self.x[idx]
+            self.apply64(xin, self.chi64[idx].contiguous(), self.t64[:nb])
+            self.r[idx] = self.b[idx] - self.t64[:nb]
+            rn = torch.linalg.vector_norm(self.r[idx], dim=1)
+            self.pending_refine = (slots, rn.to("cpu", non_blocking=True))
+
+
+class Server:
+    """Continuous-batching Lippmann-Schwinger solver (module docstring).
+
+    inner: "fp32" (FP32 cycles, FP64 refinement) or "fp64" (plain FP64
+    restarted GMRES).  gs: Gram-Schmidt of the Arnoldi step, "cgs1" (one
+    classical pass), "cgs2" (two) or "dgks" (a second pass when the first
+    removes more than 1 - 1/sqrt(2) of the norm); default cgs1 for fp32
+    cycles and cgs2 for fp64.  max_batch: slots per grid.  max_active: requests in
+    flight over all grids (None: no limit; 1 gives one solve after another).
+    mem_budget: bytes of device memory the server may hold (default: 90% of
+    what is free at start)."""
+
+    def __init__(self, restart=30, inner="fp32", inner_tol=1e-4, max_batch=16, mem_budget=None, max_cycles=200,
+                 device="cuda:0", ops_factory=None, max_active=None, gs=None):
+        torch = _torch()
+        self.restart = restart
+        self.inner = inner
+        self.inner_tol = inner_tol
+        self.max_batch = max_batch
+        self.max_cycles = max_cycles
+        self.max_active = max_active
+        # Gram-Schmidt of the Arnoldi step: one pass in FP32 cycles (their
+        # result is corrected by the FP64 residual and update), two in plain
+        # FP64 solves
+        self.gs = gs or ("cgs1" if inner == "fp32" else "cgs2")
+        if self.gs not in ("cgs1", "cgs2", "dgks"):
+            raise ValueError("gs must be cgs1, cgs2 or dgks")
+        self.device = device
+        if mem_budget is None:
+            free = torch.cuda.mem_get_info(torch.device(device))[0] if str(device).startswith("cuda") else 1 << 36
+            mem_budget = int(0.9 * free)
+        self.mem_budget = int(mem_budget)
+        self.ops_factory = ops_factory or (lambda cells, scale, frq: DeviceOps(cells, scale, frq))
+        self.ops = {}          # key -> ops (resident Green data)
+        self.grids = {}        # key -> _Grid
+        self.waiting = []
+        self.done = []
+        self.used = 0
+        self.rounds = 0
+        self.reorth_rounds = 0
+
+    # --- memory accounting --------------------------------------------------
+    def _grid_bytes(self, ops, cap):
+        slot = _Grid.slot_bytes(ops.n, ops.ncell, self.restart, self.inner)
+        return cap * (slot + ops.ws_bytes(1 if self.inner == "fp32" else 0, 1))
+
+    def _evict_idle(self, need):
+        """Free idle slabs, then idle operators, oldest first, until `need`
+        bytes fit."""
+        for key, g in sorted(self.grids.items(), key=lambda kv: kv[1].last_used):
+            if self.used + need <= self.mem_budget:
+                return
+            if g.k == 0:
+                self.used -= g.bytes
+                del self.grids[key]
+        busy = set(self.grids)
+        for key in list(self.ops):
+            if self.used + need <= self.mem_budget:
+                return
+            if key not in busy and not any(r.key == key for r in self.waiting):
+                self.used -= self.ops[key].fixed_bytes
+                del self.ops[key]
+
+    def _grid_for(self, req, allow_new=True):
+        """The request's grid with a free slot, creating the operators and the
+        slab if the budget allows; None if the request must wait."""
+        key = req.key
+        g = self.grids.get(key)
+        if g is not None:
+            return g if g.k < g.cap else None
+        if not allow_new:
+            return None
+        ops = self.ops.get(key)
+        if ops is None:
+            ops = self.ops_factory(req.cells, req.scale, req.frq)
+            self.ops[key] = ops
+            self.used += ops.fixed_bytes
+        per = self._grid_bytes(ops, 1)
+        self._evict_idle(per)
+        cap = int(min(self.max_batch, max(0, (self.mem_budget - self.used) // per)))
+        if cap < 1:
+            return None
+        g = _Grid(ops, cap, self.restart, self.inner, self.device, self.gs)
+        g.inner_tol = self.inner_tol
+        g.bytes = self._grid_bytes(ops, cap)
+        self.used += g.bytes
+        self.grids[key] = g
+        return g
+
+    # --- request flow -------------------------------------------------------
+    def submit(self, req, now=None):
+        req.t_arrive = time.perf_counter() if now is None else now
+        self.waiting.append(req)
+
+    def _complete(self, g, s, now, converged):
+        torch = _torch()
+        req = g.reqs[s]
+        with g.ctx():
+            xh = g.x[s].to("cpu", copy=True)
+        if g.stream is not None:
+            g.stream.synchronize()
+        req.result = xh.numpy().reshape(req.cells + (3,), order="F")
+        req.relres = float(g.rn[s] / g.bn[s])
+        req.converged = converged
+        req.t_done = now
+        self.done.append(req)
+        g.remove(s)
+        if getattr(self, "on_done", None) is not None:
+            self.on_done(req)
+        del torch
+
+    def _admit(self, now):
+        """Admit waiting requests in arrival order.  A request whose grid is
+        full waits without holding back requests for other grids; once a
+        request cannot get a new slab, later ones may only take free slots of
+        existing grids (no overtaking for memory)."""
+        allow_new = True
+        keep = []
+        for req in self.waiting:
+            if self.max_active is not None and self.active >= self.max_active:
+                keep.append(req)
+                continue
+            g = self._grid_for(req, allow_new)
+            if g is None:
+                if req.key not in self.grids:
+                    allow_new = False
+                keep.append(req)
+                continue
+            if not g.admit(req, now):
+                req.relres = 0.0
+                self._complete(g, g.k - 1, time.perf_counter(), True)
+        self.waiting = keep
+
+    @property
+    def active(self):
+        return sum(g.k for g in self.grids.values())
+
+    def round(self):
+        """Admit what fits, then one Arnoldi step for every active request."""
+        torch = _torch()
+        self._admit(time.perf_counter())
+        grids = [g for g in self.grids.values() if g.k > 0]
+        if not grids:
+            return 0
+        for g in grids:
+            g.last_used = time.perf_counter()
+            g.launch_step()
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+        self.rounds += 1
+        if any([g.launch_reorth() for g in grids]):
+            self.reorth_rounds += 1
+            if torch.cuda.is_available():
+                torch.cuda.synchronize()
+        ends = {}
+        for g in grids:
+            ends[id(g)] = g.finish_step()
+        refine = [(g, ends[id(g)]) for g in grids if ends[id(g)]]
+        for g, slots in refine:
+            g.launch_refine(slots)
+        if refine and torch.cuda.is_available():
+            torch.cuda.synchronize()
+        now = time.perf_counter()
+        for g, slots in refine:
+            slots, rn = g.pending_refine
+            rn = rn.numpy()
+            finished = []
+            for s, v in zip(slots, rn):
+                g.rn[s] = float(v)
+                req = g.reqs[s]
+                rel = g.rn[s] / g.bn[s]
+                if rel <= req.tol:
+                    finished.append((s, True))
+                elif req.cycles >= self.max_cycles:
+                    finished.append((s, False))
+                else:
+                    with g.ctx():
+                        g.start_cycle(s)
+            # remove from the highest slot down so compaction keeps indices valid
+            for s, conv in sorted(finished, reverse=True):
+                self._complete(g, s, now, conv)
+        return len(grids)
+
+    def run(self, requests, arrivals=None, timeout=None, on_done=None):
+        """Serve `requests`; arrivals[i] (seconds from the start) is when
+        request i is submitted (all at once if None); on_done(req) is called
+        as each request completes.  Returns the completed requests in
+        completion order."""
+        self.on_done = on_done
+        t0 = time.perf_counter()
+        order = list(range(len(requests)))
+        if arrivals is not None:
+            order.sort(key=lambda i: arrivals[i])
+        nxt = 0
+        while len(self.done) < len(requests):
+            now = time.perf_counter() - t0
+            while nxt < len(order) and (arrivals is None or arrivals[order[nxt]] <= now):
+                i = order[nxt]
+                self.submit(requests[i], t0 + (0.0 if arrivals is None else arrivals[i]))
+                nxt += 1
+            if self.active == 0 and not self.waiting:
+                if nxt < len(order):
+                    time.sleep(max(0.0, arrivals[order[nxt]] - (time.perf_counter() - t0)))
+                continue
+            if self.round() == 0 and self.waiting and self.active == 0:
+                raise RuntimeError("a request does not fit the memory budget")
+            if timeout is not None and time.perf_counter() - t0 > timeout:
+                raise TimeoutError("serving did not finish in time")
+        return self.done
+
+
+def latency_stats(reqs):
+    lat = np.array([r.latency for r in reqs])
+    span = max(r.t_done for r in reqs) - min(r.t_arrive for r in reqs)
+    return {"n": len(reqs), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "solves_per_s": len(reqs) / span if span > 0 else float("inf"), "span_s": span}
+
+
+def random_request(seed, cells, scale=Fraction(1, 32), tol=1e-6):
+    """A served-stream request: chi = 0 outside a random axis-aligned box
+    covering 10-40% of the domain and Re chi ~ U[0.5, 3], Im chi ~ U[0, 0.1]
+    per cell inside; E_inc a unit plane wave with a random direction and a
+    random polarisation perpendicular to it."""
+    from .inputs import plane_wave, random_unit, uniform
+    u = uniform(seed, 16)
+    frac = 0.1 + 0.3 * u[0]
+    side = [max(1, min(c, int(round(c * frac ** (1.0 / 3.0))))) for c in cells]
+    lo = [int(u[1 + d] * (c - s + 1)) for d, (c, s) in enumerate(zip(cells, side))]
+    chi = np.zeros(cells, np.complex128, order="F")
+    box = tuple(slice(a, a + s) for a, s in zip(lo, side))
+    nbox = int(np.prod(side))
+    uv = uniform(seed + 1, 2 * nbox)
+    chi[box] = (0.5 + 2.5 * uv[:nbox] + 0.1j * uv[nbox:]).reshape(side, order="F")
+    khat = random_unit(seed + 2)
+    p = random_unit(seed + 3)
+    p = p - khat * (p @ khat)
+    p /= np.linalg.norm(p)
+    e = plane_wave(cells, (scale,) * 3, khat, p)
+    return SolveRequest(cells, chi, e, tol=tol, scale=scale)

@@ -98,6 +98,91 @@ class LippmannSchwinger:
         return sub
 
 
+class KrylovOps:
+    """Batched Gram-Schmidt products on device-resident bases through the
+    library's Krylov kernels (gla_krylov_dots / gla_krylov_axpy).  One
+    instance per stream: it owns the partial-sum workspace of the dots."""
+
+    def __init__(self):
+        self.part = None
+
+    @staticmethod
+    def _code(t):
+        torch = _torch()
+        return capi.GLA_C64 if t.dtype == torch.complex64 else capi.GLA_C128
+
+    def dots(self, V, k, nv, w, h, stream):
+        """h[r, i] = <V[r, i], w[r]> for r < k, i < nv (h complex128 (k, nv))."""
+        torch = _torch()
+        n = V.shape[-1]
+        need = capi.lib().gla_krylov_dots_workspace(k, nv, n)
+        if self.part is None or self.part.numel() * 16 < need:
+            self.part = torch.empty(need // 16 + 1, dtype=torch.complex128, device=V.device)
+        capi.check(capi.lib().gla_krylov_dots(self._code(V), k, nv, n, ctypes.c_void_p(V.data_ptr()), V.stride(0),
+                                              V.stride(1), ctypes.c_void_p(w.data_ptr()), w.stride(0),
+                                              ctypes.c_void_p(self.part.data_ptr()), self.part.numel() * 16,
+                                              ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(stream)))
+
+    def axpy(self, V, k, nv, h, w, s, stream):
+        """w[r] += s * sum_i h[r, i] V[r, i] for r < k, i < nv."""
+        n = V.shape[-1]
+        capi.check(capi.lib().gla_krylov_axpy(self._code(V), k, nv, n, ctypes.c_void_p(V.data_ptr()), V.stride(0),
+                                              V.stride(1), ctypes.c_void_p(h.data_ptr()), h.stride(0),
+                                              ctypes.c_void_p(w.data_ptr()), w.stride(0), float(s),
+                                              ctypes.c_void_p(stream)))
+
+
+def _stream():
+    torch = _torch()
+    return torch.cuda.current_stream().cuda_stream if torch.cuda.is_available() else 0
+
+
+# DGKS criterion: a Gram-Schmidt pass that leaves less than 1/sqrt(2) of the
+# vector's norm is repeated (the reorthogonalization test of IterativeSolvers'
+# gmres, the solver the reference's examples use)
+DGKS = 0.7071067811865476
+
+
+def gs_pass(kry, V, k, nv, w, stream):
+    """One classical Gram-Schmidt pass of w[:k] against V[:k, :nv]; returns
+    the coefficients (complex128 (k, nv)).  On the CPU (tests of the host
+    logic) the same products run in torch."""
+    torch = _torch()
+    if not V.is_cuda:
+        Vj = V[:k, :nv].to(torch.complex128)
+        h = torch.einsum("rin,rn->ri", Vj.conj(), w[:k].to(torch.complex128))
+        w[:k] -= torch.einsum("ri,rin->rn", h, Vj).to(w.dtype)
+        return h
+    h = torch.empty((k, nv), dtype=torch.complex128, device=V.device)
+    kry.dots(V, k, nv, w, h, stream)
+    kry.axpy(V, k, nv, h, w, -1.0, stream)
+    return h
+
+
+def orthogonalize(kry, V, k, nv, w, stream):
+    """Gram-Schmidt of w[:k] against V[:k, :nv] with a DGKS-tested second
+    pass; returns (coefficients complex128 (k, nv), norms of the result as a
+    host array).  Synchronizes with the device to read the norms."""
+    torch = _torch()
+    n0 = torch.linalg.vector_norm(w[:k], dim=1)
+    h = gs_pass(kry, V, k, nv, w, stream)
+    n1 = torch.linalg.vector_norm(w[:k], dim=1)
+    a0, a1 = n0.double().cpu().numpy(), n1.double().cpu().numpy()
+    if (a1 < DGKS * a0).any():
+        h = h + gs_pass(kry, V, k, nv, w, stream)
+        a1 = torch.linalg.vector_norm(w[:k], dim=1).double().cpu().numpy()
+    return h, a1
+
+
+def combine(kry, V, r, nv, y, out, stream):
+    """out += sum_i y[i] V[r, i] (y complex128 (nv,), out (n,) of V's type)."""
+    torch = _torch()
+    if not V.is_cuda:
+        out += (y.to(torch.complex128) @ V[r, :nv].to(torch.complex128)).to(out.dtype)
+        return
+    kry.axpy(V[r:r + 1], 1, nv, y.reshape(1, -1).contiguous(), out.reshape(1, -1), 1.0, stream)
+
+
 class SolveInfo:
     def __init__(self, nrhs):
         self.converged = np.zeros(nrhs, bool)
@@ -130,18 +215,14 @@ def _arnoldi_cycle(A, r0, m, inner_tol):
     active = np.ones(nrhs, bool)
     steps = 0
     w = torch.empty((nrhs, n), dtype=dt, device=r0.device)
+    kry = KrylovOps()
+    st = _stream()
     for j in range(m):
         A.apply(V[:, j], w)
         steps += 1
-        Vj = V[:, :j + 1]                                            # (nrhs, j+1, n)
-        h = torch.bmm(Vj.conj(), w[:, :, None])                      # (nrhs, j+1, 1)
-        w -= torch.bmm(h.transpose(1, 2).contiguous(), Vj)[:, 0]
-        h2 = torch.bmm(Vj.conj(), w[:, :, None])
-        w -= torch.bmm(h2.transpose(1, 2).contiguous(), Vj)[:, 0]
-        hn = torch.linalg.vector_norm(w, dim=1)
-        hcol = (h + h2)[:, :, 0].to(torch.complex128).cpu().numpy()
-        hnn = hn.double().cpu().numpy()
-        V[:, j + 1] = w / torch.clamp(hn, min=1e-30)[:, None]
+        h, hnn = orthogonalize(kry, V, nrhs, j + 1, w, st)              # (nrhs, j+1)
+        hcol = h.cpu().numpy()
+        V[:, j + 1] = w / torch.from_numpy(np.maximum(hnn, 1e-30)).to(w.device, w.real.dtype)[:, None]
         H[:, :j + 1, j] = hcol
         H[:, j + 1, j] = hnn
         for r in range(nrhs):
@@ -174,8 +255,9 @@ def _arnoldi_cycle(A, r0, m, inner_tol):
     for r in range(nrhs):
         R = H[r, :k, :k]
         y[r] = _back_substitute(R, g[r, :k])
-    yt = torch.from_numpy(y.astype(np.complex64 if dt == torch.complex64 else np.complex128)).to(r0.device)
-    d = torch.bmm(yt[:, None, :], V[:, :k])[:, 0]
+    d = torch.zeros((nrhs, n), dtype=dt, device=r0.device)
+    for r in range(nrhs):
+        combine(kry, V, r, k, torch.from_numpy(y[r]).to(r0.device), d[r], st)
     return d, steps
 
 

@@ -179,6 +179,30 @@ gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int6
 /* Bytes of device work memory the handle holds for nrhs right-hand sides. */
 int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs);
 
+/* ---- Batched Krylov-basis reductions (GMRES building blocks) ------------
+ * No counterpart in the reference: it has no solver, and its examples pass
+ * the operator to IterativeSolvers' gmres / bicgstabl on the host
+ * (examples/scattering_guide.jl:65-68, docs/src/usage.md:370-391).  For
+ * callers running batched Gram-Schmidt on device-resident bases.  k right-
+ * hand sides, nv basis vectors of n complex elements each (dtype GLA_C64 or
+ * GLA_C128); strides in elements.  h is complex double, h[r*nv + i]
+ * (gla_krylov_dots) or h[r*h_rhs_stride + i] (gla_krylov_axpy).  Work goes to
+ * `stream` on the current device. */
+
+/* Bytes of the partial-sum workspace gla_krylov_dots needs. */
+int64_t gla_krylov_dots_workspace(int64_t k, int64_t nv, int64_t n);
+
+/* h[r][i] = sum_q conj(V[r][i][q]) w[r][q], summed in FP64 in a fixed order
+ * (the result for one right-hand side does not depend on k). */
+gla_status gla_krylov_dots(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
+                           int64_t v_vec_stride, const void* w, int64_t w_rhs_stride, void* part,
+                           int64_t part_bytes, void* h, void* stream);
+
+/* w[r][q] += s * sum_i h[r][i] V[r][i][q]. */
+gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
+                           int64_t v_vec_stride, const void* h, int64_t h_rhs_stride, void* w,
+                           int64_t w_rhs_stride, double s, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

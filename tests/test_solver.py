@@ -159,3 +159,37 @@ def test_gmres_ir_on_gpu(gila, orc, gpu):
     assert info.converged.all(), info
     for r in range(2):
         assert _oracle_residual(orc, N, fur, media[r], x[r].cpu().numpy(), b[r].cpu().numpy()) <= 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt,tol", [("complex64", 1e-6), ("complex128", 1e-13)])
+def test_krylov_kernels_match_host(gila, gpu, dt, tol):
+    import torch
+    print(f"case krylov kernels {dt}", flush=True)
+    tdt = getattr(torch, dt)
+    cap, m1, n, k, nv = 4, 7, 150001, 3, 5
+    g = torch.Generator().manual_seed(3)
+    V = (torch.randn((cap, m1, n), generator=g, dtype=torch.float64)
+         + 1j * torch.randn((cap, m1, n), generator=g, dtype=torch.float64)).to(tdt)
+    w = (torch.randn((cap, n), generator=g, dtype=torch.float64)
+         + 1j * torch.randn((cap, n), generator=g, dtype=torch.float64)).to(tdt)
+    ref = torch.einsum("rin,rn->ri", V[:k, :nv].conj().to(torch.complex128), w[:k].to(torch.complex128))
+    kry = gila.solve.KrylovOps()
+    Vd, wd = V.cuda(), w.cuda()
+    h = torch.empty((k, nv), dtype=torch.complex128, device="cuda")
+    kry.dots(Vd, k, nv, wd, h, 0)
+    torch.cuda.synchronize()
+    assert (h.cpu() - ref).abs().max() <= tol * ref.abs().max()
+    # one right-hand side alone gives bitwise the same coefficients
+    h1 = torch.empty((1, nv), dtype=torch.complex128, device="cuda")
+    kry.dots(Vd[1:], 1, nv, wd[1:], h1, 0)
+    torch.cuda.synchronize()
+    assert torch.equal(h1.cpu()[0], h.cpu()[1])
+    # w -= V h recovers w orthogonal to the basis span's projection
+    upd = wd.clone()
+    kry.axpy(Vd, k, nv, h, upd, -1.0, 0)
+    torch.cuda.synchronize()
+    want = w[:k].to(torch.complex128) - torch.einsum("ri,rin->rn", ref, V[:k, :nv].to(torch.complex128))
+    err = (upd[:k].cpu().to(torch.complex128) - want).abs().max() / want.abs().max()
+    assert err <= (1e-5 if dt == "complex64" else 1e-12)
+    assert torch.equal(upd[k:].cpu(), w[k:])          # rows beyond k untouched
