@@ -41,6 +41,11 @@ def parse():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched right-hand-side leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served solve-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=30)
+    ap.add_argument("--serve-rate", type=float, default=4.0,
+                    help="Poisson arrival rate of the served stream, in multiples of the sequential FP64 rate")
     ap.add_argument("--counters", choices=["c64", "c128"], default=None,
                     help="short program for rocprofv3 counter passes: a 512 MiB device copy (the calibration "
                          "of FETCH_SIZE/WRITE_SIZE) and three matvecs of the given type; prints no JSON")
@@ -115,6 +120,169 @@ def profile(gila, mem, xd, yd, reps=20):
     torch.cuda.synchronize()
     names = ["x_forward", "y_forward", "z_fused", "y_inverse", "x_inverse"]
     return [{"kernel": n, "ms": ms[i], "bytes": by[i], "gbs": by[i] / (ms[i] * 1e6)} for i, n in enumerate(names)]
+
+
+def event_ms(fn, reps, stream):
+    """Average milliseconds of fn() over reps calls, by events on `stream`."""
+    import torch
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(reps):
+        fn()
+    t1.record(stream)
+    t1.synchronize()
+    return t0.elapsed_time(t1) / reps
+
+
+def batched_leg(gila, log):
+    """96^3 self operator, 64 right-hand sides in one launch (32 plane waves,
+    32 point dipoles), and a mixed batch of 16 operators replayed from one
+    HIP graph."""
+    import torch
+    from fractions import Fraction
+    from gila_amd import _capi, inputs
+    out = {}
+    N = (96, 96, 96)
+    nrhs = 64
+    log("batched: setup 96^3 c64")
+    mem = make_operator(gila, N, np.complex64)
+    fields = []
+    for i in range(32):
+        khat = inputs.random_unit(7000 + i)
+        p = inputs.random_unit(8000 + i)
+        p = p - khat * (p @ khat)
+        fields.append(inputs.plane_wave(N, (Fraction(1, 32),) * 3, khat, p / np.linalg.norm(p)))
+    u = inputs.uniform(9000, 3 * 32)
+    for i in range(32):
+        cell = [int(u[3 * i + d] * N[d]) for d in range(3)]
+        fields.append(inputs.point_dipole(N, cell, inputs.random_unit(9100 + i)))
+    x = np.stack([f.reshape(-1, order="F") for f in fields]).astype(np.complex64)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    st = torch.cuda.current_stream()
+    lib = gila.lib()
+
+    def launch(n=nrhs, xx=xd, yy=yd):
+        _capi.check(lib.gla_opr_apply(mem._h, ctypes.c_void_p(xx.data_ptr()), xx[:n].numel(),
+                                      ctypes.c_void_p(yy.data_ptr()), yy[:n].numel(), n,
+                                      ctypes.c_void_p(st.cuda_stream)))
+    launch()
+    torch.cuda.synchronize()
+    ms = event_ms(launch, 5, st)
+    per_rhs = 8 * (78 * 96 ** 3 + 6 * 97 ** 3)
+    # RHS 0 and 63 alone give bitwise the batch's columns
+    y1 = torch.empty_like(yd[:1])
+    same = True
+    for r in (0, nrhs - 1):
+        launch(1, xd[r:r + 1], y1)
+        torch.cuda.synchronize()
+        same = same and torch.equal(y1[0], yd[r])
+    out["rhs64"] = {"workload": "96^3 self operator, ComplexF32, 64 right-hand sides (32 plane waves, 32 point "
+                                "dipoles) in one launch",
+                    "ms_per_launch": ms, "rhs_matvecs_per_s": nrhs * 1000.0 / ms,
+                    "hbm_gbs_alg": nrhs * per_rhs / (ms * 1e6),
+                    "ideal_ms_green_once_8tbs": (nrhs * (per_rhs - 8 * 6 * 97 ** 3) + 8 * 6 * 97 ** 3) / 8e9,
+                    "single_rhs_bitwise_equal": bool(same)}
+    del xd, yd, y1, mem
+    torch.cuda.empty_cache()
+
+    shapes = [(24, 24, 24), (32, 48, 64), (64, 64, 64), (80, 40, 40)]
+    uu = inputs.uniform(4242, 16)
+    ops = []
+    for i in range(16):
+        cells = shapes[int(uu[i] * len(shapes))]
+        m = make_operator(gila, cells, np.complex64)
+        xi = torch.from_numpy(inputs.complex_gaussian(cells + (3,), seed=500 + i).astype(np.complex64)
+                              .reshape(-1, order="F").copy()).cuda()
+        ops.append((cells, m, xi, torch.empty_like(xi)))
+    log("batched: mixed batch of 16 operators " + ",".join("x".join(map(str, o[0])) for o in ops))
+
+    def seq(stream):
+        for cells, m, xi, yi in ops:
+            _capi.check(lib.gla_opr_apply(m._h, ctypes.c_void_p(xi.data_ptr()), xi.numel(),
+                                          ctypes.c_void_p(yi.data_ptr()), yi.numel(), 1,
+                                          ctypes.c_void_p(stream.cuda_stream)))
+    seq(st)
+    torch.cuda.synchronize()
+    eager = [o[3].clone() for o in ops]
+    ms_eager = event_ms(lambda: seq(st), 10, st)
+    g = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(st)
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(g, stream=cap):
+            seq(cap)
+    st.wait_stream(cap)
+    torch.cuda.synchronize()
+    ok = True
+    for _ in range(3):
+        for o in ops:
+            o[3].zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        ok = ok and all(torch.equal(o[3], e) for o, e in zip(ops, eager))
+    ms_graph = event_ms(g.replay, 10, st)
+    out["mixed16"] = {"workload": "16 self operators drawn (seeded) from 24^3, 32x48x64, 64^3, 80x40x40, "
+                                  "ComplexF32, one matvec each",
+                      "shapes": ["x".join(map(str, o[0])) for o in ops],
+                      "ms_eager": ms_eager, "ms_hipgraph": ms_graph,
+                      "matvecs_per_s_hipgraph": 16 * 1000.0 / ms_graph,
+                      "graph_replays_bitwise_equal_eager": bool(ok)}
+    del ops, g
+    torch.cuda.empty_cache()
+    return out
+
+
+def serve_leg(gila, a, log):
+    """Served solve stream against sequential plain FP64 solves of the same
+    requests (GMRES(30) on (I - chi G) E = E_inc to 1e-6 by the FP64
+    residual)."""
+    S = gila.serve
+    shapes = [(32, 32, 32), (64, 64, 48), (128, 64, 64), (128, 128, 128), (256, 128, 64)]
+    from gila_amd import inputs
+    u = inputs.uniform(31337, 2 * a.serve_requests)
+
+    def requests():
+        return [S.random_request(20000 + i, shapes[int(u[i] * len(shapes))], tol=1e-6)
+                for i in range(a.serve_requests)]
+    log(f"serve: operators for {len(shapes)} grids (setup, not timed)")
+    seq = S.Server(inner="fp64", max_batch=1, max_active=1, max_cycles=60)
+    srv = S.Server(inner="fp32", max_batch=16, max_cycles=60)
+    for sh in shapes:
+        seq._grid_for(S.random_request(1, sh))
+        srv._grid_for(S.random_request(1, sh))
+    reqs = requests()
+    log(f"serve: {len(reqs)} sequential plain FP64 solves")
+    t0 = time.perf_counter()
+    seq.run(reqs, on_done=lambda r: log(f"  seq {r.rid} {'x'.join(map(str, r.cells))} steps={r.steps}"))
+    t_seq = time.perf_counter() - t0
+    rate_seq = len(reqs) / t_seq
+    lam = a.serve_rate * rate_seq
+    gaps = -np.log(u[a.serve_requests:]) / lam
+    arrivals = np.cumsum(gaps) - gaps[0]
+    reqs2 = requests()
+    log(f"serve: served stream, Poisson arrivals at {lam:.2f}/s ({a.serve_rate}x the sequential rate)")
+    srv.run(reqs2, arrivals=list(arrivals),
+            on_done=lambda r: log(f"  srv {r.rid} {'x'.join(map(str, r.cells))} steps={r.steps} "
+                                  f"lat={r.latency:.2f}s"))
+    st = S.latency_stats(reqs2)
+    st_seq = S.latency_stats(reqs)
+    agree = max(float(np.linalg.norm(r2.result - r1.result) / np.linalg.norm(r1.result))
+                for r1, r2 in zip(reqs, reqs2))
+    return {"workload": f"{len(reqs)} requests, domains drawn (seeded) from "
+                        + ", ".join("x".join(map(str, s)) for s in shapes)
+                        + " cells at 1/32 wavelength; chi 0 outside a random box of 10-40% of the domain, "
+                          "U[0.5,3]+i U[0,0.1] inside; E_inc a plane wave at a random angle",
+            "solves_per_s": st["solves_per_s"], "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
+            "arrival_rate_per_s": lam,
+            "sequential_fp64_solves_per_s": rate_seq, "sequential_fp64_p50_latency_s": st_seq["p50_s"],
+            "speedup_vs_sequential_fp64": st["solves_per_s"] / rate_seq,
+            "max_rel_diff_served_vs_sequential": agree,
+            "all_converged": bool(all(r.converged for r in reqs + reqs2)),
+            "max_relres": max(r.relres for r in reqs + reqs2),
+            "steps_total": sum(r.steps for r in reqs2)}
 
 
 def main():
@@ -210,6 +378,12 @@ def main():
                                          f"{med:.3f} s"}
         out["parity"] = {"c64_rel_l2_vs_fp64_oracle": orc.rel_l2(r32["y"], y_ref),
                          "c128_rel_l2_vs_fp64_oracle": orc.rel_l2(r64["y"], y_ref)}
+    del res, r32, r64
+    torch.cuda.empty_cache()
+    if not a.no_batched:
+        out["batched"] = batched_leg(gila, log)
+    if not a.no_serve:
+        out["serving"] = serve_leg(gila, a, log)
     print(json.dumps(out), flush=True)
 
 

@@ -29,6 +29,7 @@ EXPORTED = (
     "gla_opr_export_fur", "gla_opr_get_dims", "gla_opr_adjoint", "gla_opr_destroy", "gla_opr_apply",
     "gla_opr_apply_axpby", "gla_opr_apply_host", "gla_opr_workspace_bytes", "gla_opr_profile",
     "gla_ls_apply", "gla_krylov_dots_workspace", "gla_krylov_dots", "gla_krylov_axpy",
+    "gla_krylov_rows",
 )
 
 
@@ -84,8 +85,9 @@ def load(path=LIB_PATH):
         "gla_opr_apply_host": [vp, vp, i64, vp, i64, i64],
         "gla_opr_profile": [vp, vp, i64, vp, i64, i64, vp, i32, P(d), P(i64)],
         "gla_ls_apply": [vp, vp, i64, i64, vp, i64, vp, i64, i64, vp],
-        "gla_krylov_dots": [i32, i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, vp, vp],
-        "gla_krylov_axpy": [i32, i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, d, vp],
+        "gla_krylov_dots": [i32, i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, vp, vp, vp],
+        "gla_krylov_axpy": [i32, i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, d, vp, i64, vp, vp],
+        "gla_krylov_rows": [i32, i64, i64, vp, i64, i64, vp, vp, i64, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)

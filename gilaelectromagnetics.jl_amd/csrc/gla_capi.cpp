@@ -584,8 +584,7 @@ int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs)
 int64_t gla_krylov_dots_workspace(int64_t k, int64_t nv, int64_t n)
 {
     if (k < 0 || nv < 0 || n < 0) return -1;
-    const int64_t chunk = gla::krylov_chunk(n);
-    return k * nv * ((n + chunk - 1) / chunk) * 16;
+    return gla::krylov_part_doubles(k, nv, n) * 8;
 }
 
 static gla_status krylov_check(int32_t dtype, int64_t k, int64_t nv, int64_t n)
@@ -599,34 +598,51 @@ static gla_status krylov_check(int32_t dtype, int64_t k, int64_t nv, int64_t n)
 
 gla_status gla_krylov_dots(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* w, int64_t w_rhs_stride, void* part,
-                           int64_t part_bytes, void* h, void* stream)
+                           int64_t part_bytes, void* h, double* w2, void* stream)
 {
     if (gla_status s = krylov_check(dtype, k, nv, n)) return s;
-    if (k == 0 || nv == 0) return GLA_OK;
-    if (!V || !w || !h || (!part && n > 0)) return fail(GLA_ERR_ARG, "null argument");
+    if (k == 0 || (nv == 0 && !w2)) return GLA_OK;
+    if (n == 0) return fail(GLA_ERR_ARG, "empty vectors");
+    if (!w || !part || (nv > 0 && (!V || !h))) return fail(GLA_ERR_ARG, "null argument");
     if (part_bytes < gla_krylov_dots_workspace(k, nv, n))
         return fail(GLA_ERR_SIZE, "partial-sum workspace smaller than gla_krylov_dots_workspace()");
-    if (n == 0) return fail(GLA_ERR_ARG, "empty vectors");
     const hipStream_t st = (hipStream_t)stream;
     const hipError_t e = dtype == GLA_C64
-        ? gla::run_krylov_dots<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, st)
-        : gla::run_krylov_dots<double>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, st);
+        ? gla::run_krylov_dots<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, w2, st)
+        : gla::run_krylov_dots<double>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, w2,
+                                       st);
     return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
 }
 
 gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* h, int64_t h_rhs_stride, void* w,
-                           int64_t w_rhs_stride, double s, void* stream)
+                           int64_t w_rhs_stride, double s, void* part, int64_t part_bytes, double* w2,
+                           void* stream)
 {
     if (gla_status st = krylov_check(dtype, k, nv, n)) return st;
     if (k == 0 || n == 0) return GLA_OK;
     if (!w || (nv > 0 && (!V || !h))) return fail(GLA_ERR_ARG, "null argument");
+    if (w2 && (!part || part_bytes < gla_krylov_dots_workspace(k, 0, n)))
+        return fail(GLA_ERR_SIZE, "a norm needs the partial-sum workspace (gla_krylov_dots_workspace(k, 0, n))");
     const hipStream_t hs = (hipStream_t)stream;
     const hipError_t e = dtype == GLA_C64
         ? gla::run_krylov_axpy<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, h, h_rhs_stride, w,
-                                      w_rhs_stride, s, hs)
+                                      w_rhs_stride, s, part, w2, hs)
         : gla::run_krylov_axpy<double>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, h, h_rhs_stride, w,
-                                       w_rhs_stride, s, hs);
+                                       w_rhs_stride, s, part, w2, hs);
+    return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
+}
+
+gla_status gla_krylov_rows(int32_t dtype, int64_t k, int64_t n, void* V, int64_t v_rhs_stride, int64_t v_vec_stride,
+                           const int64_t* idx, void* b, int64_t b_rhs_stride, const double* scale, void* stream)
+{
+    if (gla_status st = krylov_check(dtype, k, 0, n)) return st;
+    if (k == 0 || n == 0) return GLA_OK;
+    if (!V || !idx || !b) return fail(GLA_ERR_ARG, "null argument");
+    const hipStream_t hs = (hipStream_t)stream;
+    const hipError_t e = dtype == GLA_C64
+        ? gla::run_krylov_rows<float>(k, n, V, v_rhs_stride, v_vec_stride, idx, b, b_rhs_stride, scale, hs)
+        : gla::run_krylov_rows<double>(k, n, V, v_rhs_stride, v_vec_stride, idx, b, b_rhs_stride, scale, hs);
     return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
 }
 

@@ -1,15 +1,16 @@
 // Krylov-basis kernels for the batched Lippmann-Schwinger GMRES: block dot
-// products h[r][i] = sum_n conj(V[r][i][n]) w[r][n] and block updates
-// w[r] += s sum_i h[r][i] V[r][i], for r < k right-hand sides and i < nv
-// basis vectors of length n.  The reference has no solver (its examples use
-// IterativeSolvers' gmres on the host, docs/src/usage.md:370-391); these are
-// the reductions that a library GEMM handles badly at this shape (two tiny
-// dimensions and one of millions).
+// products h[r][i] = sum_n conj(V[r][i][n]) w[r][n], block updates
+// w[r] += s sum_i h[r][i] V[r][i] (with the squared norm of the result), and
+// the row gather / scaled scatter between a basis slab and a batch, for r < k
+// right-hand sides and i < nv basis vectors of length n.  The reference has
+// no solver (its examples use IterativeSolvers' gmres on the host,
+// docs/src/usage.md:370-391); these are the reductions a library GEMM handles
+// badly at this shape (two tiny dimensions and one of millions).
 //
-// Dots: every workgroup reduces one chunk of one right-hand side for all nv
-// vectors (w read once per vector from L2), accumulating in FP64; the chunk
-// partials are summed in a fixed order by a second kernel, so h does not
-// depend on how many right-hand sides share the launch.
+// Every reduction is two-stage: workgroups reduce fixed chunks of one
+// right-hand side (FP64 accumulators, a fixed shuffle tree), and one wave per
+// output sums the chunk partials in a fixed order.  A right-hand side's
+// results therefore do not depend on how many share the launch.
 #include <hip/hip_runtime.h>
 
 #include "gla_fft.hpp"
@@ -18,130 +19,235 @@
 namespace gla {
 namespace {
 
-constexpr int KD_THREADS = 256;
+constexpr int KT = 256;          // threads per chunk workgroup
+constexpr int KW = KT / 64;      // waves per chunk workgroup
+constexpr int KG = 8;            // basis vectors per sweep over a chunk
 
-template <typename R>
-__global__ __launch_bounds__(KD_THREADS) void k_dots(const cpx<R>* __restrict__ V, int64_t vr, int64_t vv,
-                                                     const cpx<R>* __restrict__ w, int64_t wr, int64_t n,
-                                                     int nv, int64_t chunk, int nchunk,
-                                                     cpx<double>* __restrict__ part)
+__device__ __forceinline__ double wave_sum(double v)
 {
-    __shared__ double red[2][KD_THREADS / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// part layout: [r][slot][chunk]; slots 0..nv-1 the dots, slot nv the squared
+// norm of w (when requested)
+template <typename R>
+__global__ __launch_bounds__(KT) void k_dots(const cpx<R>* __restrict__ V, int64_t vr, int64_t vv,
+                                             const cpx<R>* __restrict__ w, int64_t wr, int64_t n, int nv,
+                                             int64_t chunk, int nchunk, int nslot, int self,
+                                             double* __restrict__ part)
+{
+    __shared__ double red[2 * KG + 1][KW];
     const int r = blockIdx.y, c = blockIdx.x;
     const int64_t lo = (int64_t)c * chunk;
     const int64_t hi = lo + chunk < n ? lo + chunk : n;
     const cpx<R>* wp = w + r * wr;
     const cpx<R>* vp = V + r * vr;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int i = 0; i < nv; ++i) {
-        double ar = 0.0, ai = 0.0;
-        for (int64_t q = lo + threadIdx.x; q < hi; q += KD_THREADS) {
-            const cpx<R> a = vp[i * vv + q], b = wp[q];
-            // conj(a) * b
-            ar += (double)a.x * b.x + (double)a.y * b.y;
-            ai += (double)a.x * b.y - (double)a.y * b.x;
+    double* pr = part + (int64_t)r * nslot * 2 * nchunk;
+    for (int i0 = 0; i0 < nv || (i0 == 0 && self); i0 += KG) {
+        double acc[2 * KG + 1];
+#pragma unroll
+        for (int g = 0; g < 2 * KG + 1; ++g) acc[g] = 0.0;
+        const bool first = i0 == 0 && self;
+        for (int64_t q = lo + threadIdx.x; q < hi; q += KT) {
+            const cpx<R> b = wp[q];
+            if (first) acc[2 * KG] += (double)b.x * b.x + (double)b.y * b.y;
+#pragma unroll
+            for (int g = 0; g < KG; ++g)
+                if (i0 + g < nv) {
+                    const cpx<R> a = vp[(int64_t)(i0 + g) * vv + q];
+                    acc[2 * g] += (double)a.x * b.x + (double)a.y * b.y;        // conj(a) b
+                    acc[2 * g + 1] += (double)a.x * b.y - (double)a.y * b.x;
+                }
         }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            ar += __shfl_xor(ar, o);
-            ai += __shfl_xor(ai, o);
-        }
-        if (lane == 0) {
-            red[0][wv] = ar;
-            red[1][wv] = ai;
+        for (int g = 0; g < 2 * KG + 1; ++g) {
+            const double v = wave_sum(acc[g]);
+            if (lane == 0) red[g][wv] = v;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double sr = 0.0, si = 0.0;
+        if (threadIdx.x < 2 * KG + 1) {
+            const int g = threadIdx.x;
+            double sum = 0.0;
 #pragma unroll
-            for (int u = 0; u < KD_THREADS / 64; ++u) {
-                sr += red[0][u];
-                si += red[1][u];
+            for (int u = 0; u < KW; ++u) sum += red[g][u];
+            if (g < 2 * KG) {
+                const int i = i0 + g / 2;
+                if (i < nv) pr[((int64_t)i * 2 + (g & 1)) * nchunk + c] = sum;
+            } else if (first) {
+                pr[((int64_t)nv * 2) * nchunk + c] = sum;
             }
-            part[((int64_t)r * nv + i) * nchunk + c] = cpx<double>{sr, si};
         }
         __syncthreads();
     }
 }
 
-__global__ void k_dots_sum(const cpx<double>* __restrict__ part, int nchunk, int64_t total, cpx<double>* __restrict__ h)
+// w[r] += s sum_i h[r][i] V[r][i] over one chunk; the chunk's squared norm
+// of the result goes to part[r][0][chunk] when norm is set
+template <typename R>
+__global__ __launch_bounds__(KT) void k_axpy(const cpx<R>* __restrict__ V, int64_t vr, int64_t vv,
+                                             const cpx<double>* __restrict__ h, int64_t hr,
+                                             cpx<R>* __restrict__ w, int64_t wr, int64_t n, int nv, double s,
+                                             int64_t chunk, int nchunk, int norm, double* __restrict__ part)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= total) return;
-    double sr = 0.0, si = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
-        sr += part[j * nchunk + c].x;
-        si += part[j * nchunk + c].y;
+    __shared__ double red[KW];
+    const int r = blockIdx.y, c = blockIdx.x;
+    const int64_t lo = (int64_t)c * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    const cpx<R>* vp = V + r * vr;
+    const cpx<double>* hp = h + r * hr;
+    cpx<R>* wp = w + r * wr;
+    double nrm = 0.0;
+    for (int64_t q = lo + threadIdx.x; q < hi; q += KT) {
+        double ar = 0.0, ai = 0.0;
+        for (int i = 0; i < nv; ++i) {
+            const cpx<R> a = vp[(int64_t)i * vv + q];
+            const cpx<double> cc = hp[i];
+            ar += cc.x * a.x - cc.y * a.y;
+            ai += cc.x * a.y + cc.y * a.x;
+        }
+        const cpx<R> o = wp[q];
+        const cpx<R> res{R((double)o.x + s * ar), R((double)o.y + s * ai)};
+        wp[q] = res;
+        nrm += (double)res.x * res.x + (double)res.y * res.y;
     }
-    h[j] = cpx<double>{sr, si};
+    if (!norm) return;   // uniform over the launch
+    nrm = wave_sum(nrm);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+#pragma unroll
+        for (int u = 0; u < KW; ++u) sum += red[u];
+        part[(int64_t)r * nchunk + c] = sum;
+    }
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_axpy(const cpx<R>* __restrict__ V, int64_t vr, int64_t vv,
-                                              const cpx<double>* __restrict__ h, int64_t hr,
-                                              cpx<R>* __restrict__ w, int64_t wr, int64_t n, int nv, double s)
+// out[j] = sum over chunks of part[j][chunk], one wave per output: lane l sums
+// chunks l, l+64, ... in order, then a fixed shuffle tree
+__global__ __launch_bounds__(64) void k_chunk_sum(const double* __restrict__ part, int nchunk, int64_t total,
+                                                  double* __restrict__ out)
+{
+    const int64_t j = blockIdx.x;
+    if (j >= total) return;
+    double v = 0.0;
+    for (int c = threadIdx.x; c < nchunk; c += 64) v += part[j * nchunk + c];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) out[j] = v;
+}
+
+// out[r] = V[r][idx[r]] (row gather); with scale: V[r][idx[r]] = out[r] / scale[r]
+template <typename R, bool SCATTER>
+__global__ __launch_bounds__(256) void k_rows(cpx<R>* __restrict__ V, int64_t vr, int64_t vv,
+                                              const int64_t* __restrict__ idx, cpx<R>* __restrict__ b,
+                                              int64_t br, const double* __restrict__ scale, int64_t n)
 {
     const int r = blockIdx.y;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const cpx<R>* vp = V + r * vr + q;
-    const cpx<double>* hp = h + r * hr;
-    double ar = 0.0, ai = 0.0;
-    for (int i = 0; i < nv; ++i) {
-        const cpx<R> a = vp[i * vv];
-        const cpx<double> c = hp[i];
-        ar += c.x * a.x - c.y * a.y;
-        ai += c.x * a.y + c.y * a.x;
+    cpx<R>* row = V + r * vr + idx[r] * vv;
+    if constexpr (SCATTER) {
+        const double inv = 1.0 / scale[r];
+        const cpx<R> v = b[r * br + q];
+        row[q] = cpx<R>{R(v.x * inv), R(v.y * inv)};
+    } else {
+        b[r * br + q] = row[q];
     }
-    cpx<R>& o = w[r * wr + q];
-    o = cpx<R>{R((double)o.x + s * ar), R((double)o.y + s * ai)};
 }
 
 }  // namespace
 
 int64_t krylov_chunk(int64_t n)
 {
-    // about 2048 workgroups per right-hand side for long vectors, at least
+    // about 1024 workgroups per right-hand side for long vectors, at least
     // 4096 elements per workgroup
-    int64_t c = (n + 2047) / 2048;
+    int64_t c = (n + 1023) / 1024;
     if (c < 4096) c = 4096;
     return c;
 }
 
+int64_t krylov_part_doubles(int64_t k, int64_t nv, int64_t n)
+{
+    // chunk partials of 2 (nv + 1) rows per right-hand side, then their sums
+    const int64_t chunk = krylov_chunk(n);
+    const int64_t nchunk = (n + chunk - 1) / chunk;
+    return k * 2 * (nv + 1) * (nchunk + 1);
+}
+
 template <typename R>
 hipError_t run_krylov_dots(int64_t k, int nv, int64_t n, const void* V, int64_t vr, int64_t vv, const void* w,
-                           int64_t wr, void* part, void* h, hipStream_t st)
+                           int64_t wr, void* part, void* h, double* w2, hipStream_t st)
 {
     const int64_t chunk = krylov_chunk(n);
     const int nchunk = (int)((n + chunk - 1) / chunk);
-    dim3 grid(nchunk, (unsigned)k);
-    hipLaunchKernelGGL(k_dots<R>, grid, dim3(KD_THREADS), 0, st, (const cpx<R>*)V, vr, vv, (const cpx<R>*)w, wr, n,
-                       nv, chunk, nchunk, (cpx<double>*)part);
+    const int self = w2 != nullptr;
+    const int nslot = nv + 1;
+    hipLaunchKernelGGL(k_dots<R>, dim3(nchunk, (unsigned)k), dim3(KT), 0, st, (const cpx<R>*)V, vr, vv,
+                       (const cpx<R>*)w, wr, n, nv, chunk, nchunk, nslot, self, (double*)part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int64_t total = k * nv;
-    hipLaunchKernelGGL(k_dots_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                       (const cpx<double>*)part, nchunk, total, (cpx<double>*)h);
-    return hipGetLastError();
+    // every (r, slot, re/im) row of part sums into h (re/im interleaved,
+    // h[r][i]) -- the slot-nv rows (norms) into a scratch tail of part
+    const int64_t rows = k * nslot * 2;
+    double* sums = (double*)part + rows * nchunk;
+    hipLaunchKernelGGL(k_chunk_sum, dim3((unsigned)rows), dim3(64), 0, st, (const double*)part, nchunk, rows, sums);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // unpack: sums[(r*nslot + i)*2 + {0,1}] -> h[r*nv + i], norms -> w2[r]
+    if (nv > 0) {
+        e = hipMemcpy2DAsync(h, (size_t)nv * 16, sums, (size_t)nslot * 16, (size_t)nv * 16, (size_t)k,
+                             hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+    }
+    if (self)
+        e = hipMemcpy2DAsync(w2, sizeof(double), sums + (int64_t)nv * 2, (size_t)nslot * 16, sizeof(double),
+                             (size_t)k, hipMemcpyDeviceToDevice, st);
+    return e;
 }
 
 template <typename R>
 hipError_t run_krylov_axpy(int64_t k, int nv, int64_t n, const void* V, int64_t vr, int64_t vv, const void* h,
-                           int64_t hr, void* w, int64_t wr, double s, hipStream_t st)
+                           int64_t hr, void* w, int64_t wr, double s, void* part, double* w2, hipStream_t st)
+{
+    const int64_t chunk = krylov_chunk(n);
+    const int nchunk = (int)((n + chunk - 1) / chunk);
+    const int norm = w2 != nullptr;
+    hipLaunchKernelGGL(k_axpy<R>, dim3(nchunk, (unsigned)k), dim3(KT), 0, st, (const cpx<R>*)V, vr, vv,
+                       (const cpx<double>*)h, hr, (cpx<R>*)w, wr, n, nv, s, chunk, nchunk, norm, (double*)part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !norm) return e;
+    hipLaunchKernelGGL(k_chunk_sum, dim3((unsigned)k), dim3(64), 0, st, (const double*)part, nchunk, k, w2);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t run_krylov_rows(int64_t k, int64_t n, void* V, int64_t vr, int64_t vv, const int64_t* idx, void* b,
+                           int64_t br, const double* scale, hipStream_t st)
 {
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)k);
-    hipLaunchKernelGGL(k_axpy<R>, grid, dim3(256), 0, st, (const cpx<R>*)V, vr, vv, (const cpx<double>*)h, hr,
-                       (cpx<R>*)w, wr, n, nv, s);
+    if (scale)
+        hipLaunchKernelGGL((k_rows<R, true>), grid, dim3(256), 0, st, (cpx<R>*)V, vr, vv, idx, (cpx<R>*)b, br,
+                           scale, n);
+    else
+        hipLaunchKernelGGL((k_rows<R, false>), grid, dim3(256), 0, st, (cpx<R>*)V, vr, vv, idx, (cpx<R>*)b, br,
+                           scale, n);
     return hipGetLastError();
 }
 
 template hipError_t run_krylov_dots<float>(int64_t, int, int64_t, const void*, int64_t, int64_t, const void*,
-                                           int64_t, void*, void*, hipStream_t);
+                                           int64_t, void*, void*, double*, hipStream_t);
 template hipError_t run_krylov_dots<double>(int64_t, int, int64_t, const void*, int64_t, int64_t, const void*,
-                                            int64_t, void*, void*, hipStream_t);
+                                            int64_t, void*, void*, double*, hipStream_t);
 template hipError_t run_krylov_axpy<float>(int64_t, int, int64_t, const void*, int64_t, int64_t, const void*,
-                                           int64_t, void*, int64_t, double, hipStream_t);
+                                           int64_t, void*, int64_t, double, void*, double*, hipStream_t);
 template hipError_t run_krylov_axpy<double>(int64_t, int, int64_t, const void*, int64_t, int64_t, const void*,
-                                            int64_t, void*, int64_t, double, hipStream_t);
+                                            int64_t, void*, int64_t, double, void*, double*, hipStream_t);
+template hipError_t run_krylov_rows<float>(int64_t, int64_t, void*, int64_t, int64_t, const int64_t*, void*,
+                                           int64_t, const double*, hipStream_t);
+template hipError_t run_krylov_rows<double>(int64_t, int64_t, void*, int64_t, int64_t, const int64_t*, void*,
+                                            int64_t, const double*, hipStream_t);
 
 }  // namespace gla

@@ -233,37 +233,38 @@ class _Grid:
     # --- one Arnoldi step for all active slots -------------------------------
     def launch_step(self):
         """Gather each slot's current basis vector, apply A to the batch and
-        run one classical Gram-Schmidt pass over the slab's columns 0..jm
-        (zero beyond each slot's own step); the coefficients and the norms
-        before and after the pass are copied to the host."""
+        run classical Gram-Schmidt over the slab's columns 0..jm (zero beyond
+        each slot's own step); the coefficients and the norms before and
+        after are copied to the host."""
         torch = _torch()
         k = self.k
         jm = int(self.j[:k].max())
+        st = self.stream_handle()
         with self.ctx():
             jt = torch.from_numpy(self.j[:k].copy()).to(self.dev)
             ar = torch.arange(k, device=self.dev)
-            self.Xin[:k] = self.V[ar, jt]
+            if self.V.is_cuda:
+                self.kry.rows(self.V, k, jt, self.Xin, st)
+            else:
+                self.Xin[:k] = self.V[ar, jt]
             self.apply_inner(self.Xin[:k], self.chi_in[:k], self.W[:k])
-            n0 = torch.linalg.vector_norm(self.W[:k], dim=1)
-            h = gs_pass(self.kry, self.V, k, jm + 1, self.W, self.stream_handle())
+            h, n0, n1 = gs_pass(self.kry, self.V, k, jm + 1, self.W, st)
             if self.gs == "cgs2":
-                h = h + gs_pass(self.kry, self.V, k, jm + 1, self.W, self.stream_handle())
-            n1 = torch.linalg.vector_norm(self.W[:k], dim=1)
-            host = [t.to("cpu", non_blocking=True) for t in (h, n0.double(), n1.double())]
+                h2, _, n1 = gs_pass(self.kry, self.V, k, jm + 1, self.W, st)
+                h = h + h2
+            host = [t.to("cpu", non_blocking=True) for t in (h, n0, n1)]
             self.pending = (host, jt, ar, jm)
 
     def launch_reorth(self):
-        """The second Gram-Schmidt pass: always ("cgs2"), never ("cgs1"), or
-        when a slot lost more than 1 - 1/sqrt(2) of its norm ("dgks", which
-        costs a host round trip).  Returns whether one was launched."""
-        torch = _torch()
+        """The second Gram-Schmidt pass when gs is "dgks" and a slot lost more
+        than 1 - 1/sqrt(2) of its norm (cgs2 ran it in launch_step already).
+        Returns whether one was launched."""
         (h, n0, n1), jt, ar, jm = self.pending
         self.reorth = None
-        if self.gs != "dgks" or (self.gs == "dgks" and not (n1.numpy() < DGKS * n0.numpy()).any()):
+        if self.gs != "dgks" or not (np.sqrt(n1.numpy()) < DGKS * np.sqrt(n0.numpy())).any():
             return False
         with self.ctx():
-            h2 = gs_pass(self.kry, self.V, self.k, jm + 1, self.W, self.stream_handle())
-            n2 = torch.linalg.vector_norm(self.W[:self.k], dim=1).double()
+            h2, _, n2 = gs_pass(self.kry, self.V, self.k, jm + 1, self.W, self.stream_handle())
             self.reorth = (h2.to("cpu", non_blocking=True), n2.to("cpu", non_blocking=True))
         return True
 
@@ -276,15 +277,18 @@ class _Grid:
         k = self.k
         m = self.m
         hcol = h.numpy()
-        hn = n1.numpy()
+        hn = np.sqrt(n1.numpy())
         if self.reorth is not None:
             h2, n2 = self.reorth
             self.reorth = None
             hcol = hcol + h2.numpy()
-            hn = n2.numpy()
+            hn = np.sqrt(n2.numpy())
         with self.ctx():
-            scale = torch.from_numpy(np.maximum(hn, 1e-30)).to(self.dev, self.W.real.dtype)
-            self.V[ar, jt + 1] = self.W[:k] / scale.unsqueeze(1)
+            scale = torch.from_numpy(np.maximum(hn, 1e-30)).to(self.dev)
+            if self.V.is_cuda:
+                self.kry.rows(self.V, k, jt + 1, self.W, self.stream_handle(), scale=scale)
+            else:
+                self.V[ar, jt + 1] = self.W[:k] / scale.to(self.W.real.dtype).unsqueeze(1)
         js = self.j[:k].copy()
         if k <= 4:
             self._rotate_scalar(hcol, hn, js, jm)
@@ -423,6 +427,9 @@ class Server:
         self.used = 0
         self.rounds = 0
         self.reorth_rounds = 0
+        # host seconds per phase of the rounds, and slot-steps run
+        self.times = dict.fromkeys(("launch", "wait_step", "reorth", "finish", "refine_launch", "wait_refine",
+                                    "complete", "slot_rounds"), 0.0)
 
     # --- memory accounting --------------------------------------------------
     def _grid_bytes(self, ops, cap):
@@ -527,25 +534,39 @@ class Server:
         grids = [g for g in self.grids.values() if g.k > 0]
         if not grids:
             return 0
+        tm = self.times
+        t0 = time.perf_counter()
         for g in grids:
-            g.last_used = time.perf_counter()
+            g.last_used = t0
             g.launch_step()
+        t1 = time.perf_counter()
         if torch.cuda.is_available():
             torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tm["launch"] += t1 - t0
+        tm["wait_step"] += t2 - t1
+        tm["slot_rounds"] += sum(g.k for g in grids)
         self.rounds += 1
         if any([g.launch_reorth() for g in grids]):
             self.reorth_rounds += 1
             if torch.cuda.is_available():
                 torch.cuda.synchronize()
+        t3 = time.perf_counter()
         ends = {}
         for g in grids:
             ends[id(g)] = g.finish_step()
         refine = [(g, ends[id(g)]) for g in grids if ends[id(g)]]
+        t4 = time.perf_counter()
         for g, slots in refine:
             g.launch_refine(slots)
+        t5 = time.perf_counter()
         if refine and torch.cuda.is_available():
             torch.cuda.synchronize()
         now = time.perf_counter()
+        tm["reorth"] += t3 - t2
+        tm["finish"] += t4 - t3
+        tm["refine_launch"] += t5 - t4
+        tm["wait_refine"] += now - t5
         for g, slots in refine:
             slots, rn = g.pending_refine
             rn = rn.numpy()
@@ -564,6 +585,7 @@ class Server:
             # remove from the highest slot down so compaction keeps indices valid
             for s, conv in sorted(finished, reverse=True):
                 self._complete(g, s, now, conv)
+        tm["complete"] += time.perf_counter() - now
         return len(grids)
 
     def run(self, requests, arrivals=None, timeout=None, on_done=None):

@@ -99,9 +99,10 @@ class LippmannSchwinger:
 
 
 class KrylovOps:
-    """Batched Gram-Schmidt products on device-resident bases through the
-    library's Krylov kernels (gla_krylov_dots / gla_krylov_axpy).  One
-    instance per stream: it owns the partial-sum workspace of the dots."""
+    """Batched Gram-Schmidt products and basis-row moves on device-resident
+    bases through the library's Krylov kernels (gla_krylov_dots /
+    gla_krylov_axpy / gla_krylov_rows).  One instance per stream: it owns the
+    partial-sum workspace."""
 
     def __init__(self):
         self.part = None
@@ -111,24 +112,43 @@ class KrylovOps:
         torch = _torch()
         return capi.GLA_C64 if t.dtype == torch.complex64 else capi.GLA_C128
 
-    def dots(self, V, k, nv, w, h, stream):
-        """h[r, i] = <V[r, i], w[r]> for r < k, i < nv (h complex128 (k, nv))."""
+    def _part(self, k, nv, n, device):
         torch = _torch()
-        n = V.shape[-1]
         need = capi.lib().gla_krylov_dots_workspace(k, nv, n)
-        if self.part is None or self.part.numel() * 16 < need:
-            self.part = torch.empty(need // 16 + 1, dtype=torch.complex128, device=V.device)
-        capi.check(capi.lib().gla_krylov_dots(self._code(V), k, nv, n, ctypes.c_void_p(V.data_ptr()), V.stride(0),
-                                              V.stride(1), ctypes.c_void_p(w.data_ptr()), w.stride(0),
-                                              ctypes.c_void_p(self.part.data_ptr()), self.part.numel() * 16,
-                                              ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(stream)))
+        if self.part is None or self.part.numel() * 8 < need:
+            self.part = torch.empty(need // 8 + 1, dtype=torch.float64, device=device)
+        return self.part
 
-    def axpy(self, V, k, nv, h, w, s, stream):
-        """w[r] += s * sum_i h[r, i] V[r, i] for r < k, i < nv."""
+    @staticmethod
+    def _ptr(t):
+        return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+    def dots(self, V, k, nv, w, h, stream, w2=None):
+        """h[r, i] = <V[r, i], w[r]> for r < k, i < nv (h complex128 (k, nv));
+        w2[r] = ||w[r]||^2 (float64 device tensor) when given."""
         n = V.shape[-1]
-        capi.check(capi.lib().gla_krylov_axpy(self._code(V), k, nv, n, ctypes.c_void_p(V.data_ptr()), V.stride(0),
-                                              V.stride(1), ctypes.c_void_p(h.data_ptr()), h.stride(0),
-                                              ctypes.c_void_p(w.data_ptr()), w.stride(0), float(s),
+        part = self._part(k, nv, n, V.device)
+        capi.check(capi.lib().gla_krylov_dots(self._code(V), k, nv, n, self._ptr(V), V.stride(0), V.stride(1),
+                                              self._ptr(w), w.stride(0), self._ptr(part), part.numel() * 8,
+                                              self._ptr(h), self._ptr(w2), ctypes.c_void_p(stream)))
+
+    def axpy(self, V, k, nv, h, w, s, stream, w2=None):
+        """w[r] += s * sum_i h[r, i] V[r, i] for r < k, i < nv; w2[r] =
+        ||w[r]||^2 afterwards when given."""
+        n = V.shape[-1]
+        part = self._part(k, 0, n, V.device) if w2 is not None else None
+        capi.check(capi.lib().gla_krylov_axpy(self._code(V), k, nv, n, self._ptr(V), V.stride(0), V.stride(1),
+                                              self._ptr(h), h.stride(0) if h is not None else 0, self._ptr(w),
+                                              w.stride(0), float(s), self._ptr(part),
+                                              0 if part is None else part.numel() * 8, self._ptr(w2),
+                                              ctypes.c_void_p(stream)))
+
+    def rows(self, V, k, idx, b, stream, scale=None):
+        """scale None: b[r] = V[r, idx[r]]; otherwise V[r, idx[r]] = b[r] / scale[r]
+        (idx int64, scale float64, both device tensors of k entries)."""
+        n = V.shape[-1]
+        capi.check(capi.lib().gla_krylov_rows(self._code(V), k, n, self._ptr(V), V.stride(0), V.stride(1),
+                                              self._ptr(idx), self._ptr(b), b.stride(0), self._ptr(scale),
                                               ctypes.c_void_p(stream)))
 
 
@@ -144,33 +164,37 @@ DGKS = 0.7071067811865476
 
 
 def gs_pass(kry, V, k, nv, w, stream):
-    """One classical Gram-Schmidt pass of w[:k] against V[:k, :nv]; returns
-    the coefficients (complex128 (k, nv)).  On the CPU (tests of the host
-    logic) the same products run in torch."""
+    """One classical Gram-Schmidt pass of w[:k] against V[:k, :nv].  Returns
+    (coefficients complex128 (k, nv), ||w||^2 before, ||w||^2 after) as
+    device tensors.  On the CPU (tests of the host logic) the same products
+    run in torch."""
     torch = _torch()
     if not V.is_cuda:
         Vj = V[:k, :nv].to(torch.complex128)
-        h = torch.einsum("rin,rn->ri", Vj.conj(), w[:k].to(torch.complex128))
+        w64 = w[:k].to(torch.complex128)
+        n0 = (w64.abs() ** 2).sum(1)
+        h = torch.einsum("rin,rn->ri", Vj.conj(), w64)
         w[:k] -= torch.einsum("ri,rin->rn", h, Vj).to(w.dtype)
-        return h
+        n1 = (w[:k].to(torch.complex128).abs() ** 2).sum(1)
+        return h, n0, n1
     h = torch.empty((k, nv), dtype=torch.complex128, device=V.device)
-    kry.dots(V, k, nv, w, h, stream)
-    kry.axpy(V, k, nv, h, w, -1.0, stream)
-    return h
+    n0 = torch.empty(k, dtype=torch.float64, device=V.device)
+    n1 = torch.empty_like(n0)
+    kry.dots(V, k, nv, w, h, stream, w2=n0)
+    kry.axpy(V, k, nv, h, w, -1.0, stream, w2=n1)
+    return h, n0, n1
 
 
 def orthogonalize(kry, V, k, nv, w, stream):
     """Gram-Schmidt of w[:k] against V[:k, :nv] with a DGKS-tested second
     pass; returns (coefficients complex128 (k, nv), norms of the result as a
     host array).  Synchronizes with the device to read the norms."""
-    torch = _torch()
-    n0 = torch.linalg.vector_norm(w[:k], dim=1)
-    h = gs_pass(kry, V, k, nv, w, stream)
-    n1 = torch.linalg.vector_norm(w[:k], dim=1)
-    a0, a1 = n0.double().cpu().numpy(), n1.double().cpu().numpy()
+    h, n0, n1 = gs_pass(kry, V, k, nv, w, stream)
+    a0, a1 = np.sqrt(n0.cpu().numpy()), np.sqrt(n1.cpu().numpy())
     if (a1 < DGKS * a0).any():
-        h = h + gs_pass(kry, V, k, nv, w, stream)
-        a1 = torch.linalg.vector_norm(w[:k], dim=1).double().cpu().numpy()
+        h2, _, n2 = gs_pass(kry, V, k, nv, w, stream)
+        h = h + h2
+        a1 = np.sqrt(n2.cpu().numpy())
     return h, a1
 
 

@@ -189,19 +189,29 @@ int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs);
  * (gla_krylov_dots) or h[r*h_rhs_stride + i] (gla_krylov_axpy).  Work goes to
  * `stream` on the current device. */
 
-/* Bytes of the partial-sum workspace gla_krylov_dots needs. */
+/* Bytes of the partial-sum workspace of gla_krylov_dots for (k, nv, n); with
+ * nv = 0 it is the size a norm in gla_krylov_axpy needs. */
 int64_t gla_krylov_dots_workspace(int64_t k, int64_t nv, int64_t n);
 
-/* h[r][i] = sum_q conj(V[r][i][q]) w[r][q], summed in FP64 in a fixed order
- * (the result for one right-hand side does not depend on k). */
+/* h[r][i] = sum_q conj(V[r][i][q]) w[r][q], and w2[r] = ||w[r]||^2 when w2 is
+ * not NULL (device array of k doubles); FP64 sums in a fixed order (the
+ * result for one right-hand side does not depend on k). */
 gla_status gla_krylov_dots(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* w, int64_t w_rhs_stride, void* part,
-                           int64_t part_bytes, void* h, void* stream);
+                           int64_t part_bytes, void* h, double* w2, void* stream);
 
-/* w[r][q] += s * sum_i h[r][i] V[r][i][q]. */
+/* w[r][q] += s * sum_i h[r][i] V[r][i][q]; then w2[r] = ||w[r]||^2 when w2 is
+ * not NULL (part is then required, part may be NULL otherwise). */
 gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* h, int64_t h_rhs_stride, void* w,
-                           int64_t w_rhs_stride, double s, void* stream);
+                           int64_t w_rhs_stride, double s, void* part, int64_t part_bytes, double* w2,
+                           void* stream);
+
+/* Row moves between a basis slab and a batch, idx[r] (device int64) the
+ * vector of right-hand side r: scale == NULL gathers b[r] = V[r][idx[r]];
+ * otherwise scatters V[r][idx[r]] = b[r] / scale[r] (device doubles). */
+gla_status gla_krylov_rows(int32_t dtype, int64_t k, int64_t n, void* V, int64_t v_rhs_stride, int64_t v_vec_stride,
+                           const int64_t* idx, void* b, int64_t b_rhs_stride, const double* scale, void* stream);
 
 #ifdef __cplusplus
 }

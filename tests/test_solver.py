@@ -167,29 +167,47 @@ def test_krylov_kernels_match_host(gila, gpu, dt, tol):
     import torch
     print(f"case krylov kernels {dt}", flush=True)
     tdt = getattr(torch, dt)
-    cap, m1, n, k, nv = 4, 7, 150001, 3, 5
+    cap, m1, n, k, nv = 4, 11, 150001, 3, 9          # nv > 8: two sweeps of the dot kernel
     g = torch.Generator().manual_seed(3)
     V = (torch.randn((cap, m1, n), generator=g, dtype=torch.float64)
          + 1j * torch.randn((cap, m1, n), generator=g, dtype=torch.float64)).to(tdt)
     w = (torch.randn((cap, n), generator=g, dtype=torch.float64)
          + 1j * torch.randn((cap, n), generator=g, dtype=torch.float64)).to(tdt)
-    ref = torch.einsum("rin,rn->ri", V[:k, :nv].conj().to(torch.complex128), w[:k].to(torch.complex128))
+    V64, w64 = V.to(torch.complex128), w.to(torch.complex128)
+    ref = torch.einsum("rin,rn->ri", V64[:k, :nv].conj(), w64[:k])
     kry = gila.solve.KrylovOps()
     Vd, wd = V.cuda(), w.cuda()
     h = torch.empty((k, nv), dtype=torch.complex128, device="cuda")
-    kry.dots(Vd, k, nv, wd, h, 0)
+    w2 = torch.empty(k, dtype=torch.float64, device="cuda")
+    kry.dots(Vd, k, nv, wd, h, 0, w2=w2)
     torch.cuda.synchronize()
     assert (h.cpu() - ref).abs().max() <= tol * ref.abs().max()
+    assert torch.allclose(w2.cpu(), (w64[:k].abs() ** 2).sum(1), rtol=1e-12, atol=0)
     # one right-hand side alone gives bitwise the same coefficients
     h1 = torch.empty((1, nv), dtype=torch.complex128, device="cuda")
     kry.dots(Vd[1:], 1, nv, wd[1:], h1, 0)
     torch.cuda.synchronize()
     assert torch.equal(h1.cpu()[0], h.cpu()[1])
-    # w -= V h recovers w orthogonal to the basis span's projection
+    # w -= V h, with the squared norm of the result
     upd = wd.clone()
-    kry.axpy(Vd, k, nv, h, upd, -1.0, 0)
+    n1 = torch.empty(k, dtype=torch.float64, device="cuda")
+    kry.axpy(Vd, k, nv, h, upd, -1.0, 0, w2=n1)
     torch.cuda.synchronize()
-    want = w[:k].to(torch.complex128) - torch.einsum("ri,rin->rn", ref, V[:k, :nv].to(torch.complex128))
+    want = w64[:k] - torch.einsum("ri,rin->rn", ref, V64[:k, :nv])
     err = (upd[:k].cpu().to(torch.complex128) - want).abs().max() / want.abs().max()
     assert err <= (1e-5 if dt == "complex64" else 1e-12)
+    assert torch.allclose(n1.cpu(), (upd[:k].cpu().to(torch.complex128).abs() ** 2).sum(1), rtol=1e-12, atol=0)
     assert torch.equal(upd[k:].cpu(), w[k:])          # rows beyond k untouched
+    # row gather and scaled scatter
+    idx = torch.tensor([4, 0, 10], dtype=torch.int64, device="cuda")
+    b = torch.empty((k, n), dtype=tdt, device="cuda")
+    kry.rows(Vd, k, idx, b, 0)
+    torch.cuda.synchronize()
+    assert all(torch.equal(b[r].cpu(), V[r, int(idx[r])]) for r in range(k))
+    sc = torch.tensor([2.0, 0.5, 4.0], dtype=torch.float64, device="cuda")
+    Vs = Vd.clone()
+    kry.rows(Vs, k, idx, wd, 0, scale=sc)
+    torch.cuda.synchronize()
+    for r in range(k):
+        got = Vs[r, int(idx[r])].cpu().to(torch.complex128)
+        assert (got - w64[r] / float(sc[r])).abs().max() <= 1e-6 * w64[r].abs().max()
