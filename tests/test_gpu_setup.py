@@ -104,3 +104,33 @@ def test_contained_source_with_reembedded_axes(gila, orc, gpu, typ, tol):
     y_ext = gila.egoOpr_(ext, x)
     y_slf = gila.egoOpr_(slf, xt)
     assert orc.rel_l2(y_ext, y_slf) <= tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5)])
+def test_guide_shaped_external_full_size(gila, orc, gpu, typ, tol):
+    # source 256x34x34 centred in a 256x256x128 target at 1/32 wavelength (the
+    # guide recipe of examples/scattering_guide.jl:102-116): B = (256,145,81),
+    # axes 2 and 3 re-embedded in 160 and 96.  The external operator must equal
+    # the target's self operator on the embedded source, cropped -- the
+    # identity of test/extSlfTest.jl:45-47 at full size.  Input: an x-polarised
+    # point dipole in a seeded random source cell plus 1e-3 complex Gaussian noise.
+    print(f"case guide-shaped external 256x34x34 in 256x256x128 {np.dtype(typ).name}", flush=True)
+    S, T = (256, 34, 34), (256, 256, 128)
+    src = gila.GlaVol(S, (H,) * 3, (0, 0, 0))
+    tgt = gila.GlaVol(T, (H,) * 3, (0, 0, 0))
+    ext = gila.GlaOprMem(gila.GlaKerOpt(True), tgt, src, setTyp=typ)
+    assert tuple(ext._dims.brn) == (256, 145, 81) and tuple(ext._dims.dev_brn) == (256, 160, 96)
+    from gila_amd import inputs
+    u = inputs.uniform(77, 3)
+    cell = [int(u[d] * S[d]) for d in range(3)]
+    x = 1e-3 * orc.complex_gaussian(S + (3,), seed=78)
+    x[tuple(cell) + (0,)] += 1.0
+    y_ext = gila.egoOpr_(ext, x.astype(typ))
+    del ext
+    slf = gila.GlaOprMem(gila.GlaKerOpt(True), tgt, setTyp=typ)
+    off = [round((a - b) / float(H)) for a, b in zip(_first_centre(src), _first_centre(tgt))]
+    xt = np.zeros(T + (3,), typ, order="F")
+    xt[off[0]:off[0] + S[0], off[1]:off[1] + S[1], off[2]:off[2] + S[2], :] = x
+    y_slf = gila.egoOpr_(slf, xt)
+    assert orc.rel_l2(y_ext, y_slf) <= tol
