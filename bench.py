@@ -319,6 +319,19 @@ def main():
         y = yd.cpu().numpy().reshape(N + (3,), order="F")
         res[typ] = dict(mem=mem, ms=ms, wall=wall, prof=prof, y=y, setup_s=setup_s)
 
+    # the host-array entry (gla_opr_apply_host: device buffers allocated,
+    # PCIe copies in and out), for the record; never the headline value
+    xh = np.asfortranarray(x.astype(np.complex64))
+    yh = np.empty_like(xh)
+    m32 = res[np.complex64]["mem"]
+    from gila_amd import _capi
+    hts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _capi.check(gila.lib().gla_opr_apply_host(m32._h, xh.ctypes.data_as(ctypes.c_void_p), xh.size,
+                                                  yh.ctypes.data_as(ctypes.c_void_p), yh.size, 1))
+        hts.append(time.perf_counter() - t0)
+    host_ms = 1000.0 * float(np.median(hts))
     r32, r64 = res[np.complex64], res[np.complex128]
     tot_bytes = sum(p["bytes"] for p in r32["prof"])
     dom = max(r32["prof"], key=lambda p: p["ms"])
@@ -348,6 +361,7 @@ def main():
                  "hbm_gbs_whole_matvec": sum(p["bytes"] for p in r64["prof"]) / (r64["ms"] * 1e6),
                  "passes": r64["prof"]},
         "fp32_over_fp64": r64["ms"] / r32["ms"],
+        "host_buffers_ms_per_matvec": host_ms,
         "wall_s_timed_region": r32["wall"],
     }
     pmc = os.path.join(ROOT, "profiles", "traffic_z_fused_128.json")
