@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <complex>
 #include <cstring>
 #include <memory>
@@ -362,6 +363,12 @@ gla_status create_common(const gla_opts* opts, const gla_vol* trg, const gla_vol
         furd[(size_t)i] = fur_dtype == GLA_C64
                               ? std::complex<double>(((const float*)fur)[2 * i], ((const float*)fur)[2 * i + 1])
                               : std::complex<double>(((const double*)fur)[2 * i], ((const double*)fur)[2 * i + 1]);
+    // the Green data in the operator's element type first, as the
+    // reference's setTyp cast stores it (src/glaMemSup.jl:148): the device
+    // copy is then the same whether the data came from the setup, from
+    // egoFur or from a file
+    if (opts->dtype == GLA_C64)
+        for (auto& v : furd) v = std::complex<double>((float)v.real(), (float)v.imag());
     auto h = std::make_unique<gla_opr_s>();
     h->opts = *opts;
     h->trg = *trg;
@@ -419,6 +426,160 @@ gla_status gla_opr_export_fur(gla_opr_t h, void* buf, int64_t len)
                                               std::to_string(n));
     std::memcpy(buf, h->fur_host->data(), h->fur_host->size());
     return GLA_OK;
+}
+
+gla_status gla_opr_get_vols(gla_opr_t h, gla_vol* trg, gla_vol* src)
+{
+    if (!h || !trg || !src) return fail(GLA_ERR_ARG, "null argument");
+    *trg = h->trg;
+    *src = h->src;
+    return GLA_OK;
+}
+
+gla_status gla_opr_get_opts(gla_opr_t h, gla_opts* opts)
+{
+    if (!h || !opts) return fail(GLA_ERR_ARG, "null argument");
+    *opts = h->opts;
+    return GLA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Green-data file: a 512-byte little-endian header, then the eight branches
+// in the export layout (the operator's element type, re/im interleaved).
+// The header carries everything that fixes the layout (volumes, branch and
+// device lengths, stored extents, symmetric axes) so a reader can check that
+// the payload belongs to the geometry it names.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char kFurMagic[8] = {'G', 'I', 'L', 'A', 'F', 'U', 'R', '1'};
+constexpr uint32_t kFurVersion = 1;
+constexpr uint32_t kFurHeader = 512;
+
+struct FurHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    int32_t dtype, int_ord, adj_mod, self_opr;
+    double frq_re, frq_im;
+    gla_vol trg, src;
+    int64_t brn[3], dev_brn[3], fur_ext[3];
+    int32_t sym[3], reserved;
+    uint64_t n_elements, fnv1a;
+};
+static_assert(sizeof(FurHeader) <= kFurHeader, "Green-data header exceeds its 512 bytes");
+
+uint64_t fnv1a64(const unsigned char* p, size_t n)
+{
+    uint64_t hsh = 1469598103934665603ull;
+    for (size_t i = 0; i < n; ++i) {
+        hsh ^= p[i];
+        hsh *= 1099511628211ull;
+    }
+    return hsh;
+}
+
+bool little_endian_host()
+{
+    const uint32_t one = 1;
+    unsigned char b;
+    std::memcpy(&b, &one, 1);
+    return b == 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+gla_status gla_opr_save(gla_opr_t h, const char* path)
+{
+    if (!h || !path) return fail(GLA_ERR_ARG, "null argument");
+    if (!little_endian_host()) return fail(GLA_ERR_UNSUPPORTED, "Green-data files are written on little-endian hosts");
+    FurHeader hd;
+    std::memset(&hd, 0, sizeof hd);
+    std::memcpy(hd.magic, kFurMagic, 8);
+    hd.version = kFurVersion;
+    hd.header_bytes = kFurHeader;
+    hd.dtype = h->dims.dtype;
+    hd.int_ord = h->opts.int_ord;
+    hd.adj_mod = h->opts.adj_mod;
+    hd.self_opr = h->dims.self_opr;
+    hd.frq_re = h->opts.frq_phz_re;
+    hd.frq_im = h->opts.frq_phz_im;
+    hd.trg = h->trg;
+    hd.src = h->src;
+    for (int a = 0; a < 3; ++a) {
+        hd.brn[a] = h->dims.brn[a];
+        hd.dev_brn[a] = h->dims.dev_brn[a];
+        hd.fur_ext[a] = h->dims.fur_ext[a];
+        hd.sym[a] = h->dims.sym[a];
+    }
+    const std::vector<char>& pay = *h->fur_host;
+    hd.n_elements = pay.size() / elem_bytes(h->dims.dtype);
+    hd.fnv1a = fnv1a64((const unsigned char*)pay.data(), pay.size());
+    std::vector<char> head(kFurHeader, 0);
+    std::memcpy(head.data(), &hd, sizeof hd);
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(GLA_ERR_ARG, std::string("cannot open ") + path + " for writing");
+    const bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size() &&
+                    std::fwrite(pay.data(), 1, pay.size(), f) == pay.size();
+    const bool closed = std::fclose(f) == 0;
+    if (!ok || !closed) return fail(GLA_ERR_ARG, std::string("short write to ") + path);
+    return GLA_OK;
+}
+
+gla_status gla_opr_load(const char* path, int32_t dtype, int32_t device, gla_opr_t* out)
+{
+    if (!path || !out) return fail(GLA_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!little_endian_host()) return fail(GLA_ERR_UNSUPPORTED, "Green-data files are read on little-endian hosts");
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(GLA_ERR_ARG, std::string("cannot open ") + path);
+    std::vector<char> head(kFurHeader);
+    FurHeader hd;
+    if (std::fread(head.data(), 1, head.size(), f) != head.size()) {
+        std::fclose(f);
+        return fail(GLA_ERR_SIZE, std::string(path) + ": shorter than a Green-data header");
+    }
+    std::memcpy(&hd, head.data(), sizeof hd);
+    auto bad = [&](const std::string& why) {
+        std::fclose(f);
+        return fail(GLA_ERR_ARG, std::string(path) + ": " + why);
+    };
+    if (std::memcmp(hd.magic, kFurMagic, 8) != 0) return bad("not a Green-data file");
+    if (hd.version != kFurVersion || hd.header_bytes != kFurHeader) return bad("unsupported Green-data version");
+    if (hd.dtype != GLA_C64 && hd.dtype != GLA_C128) return bad("unknown element type");
+    gla_dims d;
+    const gla_vol* sp = hd.self_opr ? nullptr : &hd.src;
+    if (gla_opr_dims_for(&hd.trg, sp, &d) != GLA_OK) return bad("volumes in the header are inconsistent");
+    for (int a = 0; a < 3; ++a)
+        if (d.brn[a] != hd.brn[a] || d.dev_brn[a] != hd.dev_brn[a] || d.fur_ext[a] != hd.fur_ext[a] ||
+            d.sym[a] != hd.sym[a])
+            return bad("branch layout in the header does not match its volumes");
+    const uint64_t want = 8ull * (uint64_t)(d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2]) * 6ull;
+    if (hd.n_elements != want) return bad("element count does not match the branch layout");
+    const size_t eb = hd.dtype == GLA_C64 ? 8 : 16;
+    std::vector<char> pay((size_t)want * eb);
+    const size_t got = std::fread(pay.data(), 1, pay.size(), f);
+    char extra;
+    const bool trailing = std::fread(&extra, 1, 1, f) == 1;
+    std::fclose(f);
+    if (got != pay.size()) return fail(GLA_ERR_SIZE, std::string(path) + ": truncated Green data");
+    if (trailing) return fail(GLA_ERR_SIZE, std::string(path) + ": trailing bytes after the Green data");
+    if (fnv1a64((const unsigned char*)pay.data(), pay.size()) != hd.fnv1a)
+        return fail(GLA_ERR_ARG, std::string(path) + ": checksum mismatch");
+    gla_opts o;
+    o.frq_phz_re = hd.frq_re;
+    o.frq_phz_im = hd.frq_im;
+    o.int_ord = hd.int_ord;
+    o.adj_mod = hd.adj_mod;
+    o.dtype = dtype == 0 ? hd.dtype : dtype;
+    o.device = device;
+    if (o.dtype != GLA_C64 && o.dtype != GLA_C128) return fail(GLA_ERR_TYPE, "dtype must be 0, GLA_C64 or GLA_C128");
+    gla_status s = gla_init(device);
+    if (s) return s;
+    return create_common(&o, &hd.trg, sp, d, pay.data(), hd.dtype, (int64_t)want, out);
 }
 
 gla_status gla_opr_adjoint(gla_opr_t h, gla_opr_t* out)

@@ -232,6 +232,33 @@ class GlaOprMem:
         per = U[0] * U[1] * U[2] * 6
         return [buf[b * per:(b + 1) * per].reshape(U + (6,), order="F") for b in range(8)]
 
+    def save(self, path):
+        """Write the Green data and the geometry it belongs to (the library's
+        Green-data file; the reference's preload cache uses Julia
+        serialization, examples/GilaOperators.jl:28-55)."""
+        capi.check(capi.lib().gla_opr_save(self._h, str(path).encode()))
+
+    @classmethod
+    def load(cls, path, setTyp=None, device=0):
+        """An operator from a Green-data file; setTyp None keeps the stored
+        element type."""
+        code = 0 if setTyp is None else _dtype_code(setTyp)
+        L = capi.lib()
+        h = ctypes.c_void_p()
+        capi.check(L.gla_opr_load(str(path).encode(), code, int(device), ctypes.byref(h)))
+        tv, sv = capi.gla_vol(), capi.gla_vol()
+        opts = capi.gla_opts()
+        capi.check(L.gla_opr_get_vols(h, ctypes.byref(tv), ctypes.byref(sv)))
+        capi.check(L.gla_opr_get_opts(h, ctypes.byref(opts)))
+
+        def vol(v):
+            return GlaVol(tuple(v.cel), tuple(Fraction(v.scl_num[a], v.scl_den[a]) for a in range(3)),
+                          tuple(Fraction(v.org_num[a], v.org_den[a]) for a in range(3)))
+        trg, src = vol(tv), vol(sv)
+        cmp = GlaKerOpt(complex(opts.frq_phz_re, opts.frq_phz_im), opts.int_ord, bool(opts.adj_mod), True,
+                        device=opts.device)
+        return cls(cmp, trg, src if src != trg else trg, _handle=h)
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

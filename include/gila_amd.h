@@ -176,6 +176,22 @@ gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, i
 gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, void* stream,
                            int32_t reps, double* ms, int64_t* bytes);
 
+/* The operator's target and source volumes, and its options, as created. */
+gla_status gla_opr_get_vols(gla_opr_t h, gla_vol* trg, gla_vol* src);
+gla_status gla_opr_get_opts(gla_opr_t h, gla_opts* opts);
+
+/* Green-data files: replace the reference's preload cache, which stores
+ * GlaOprMem with Julia serialization (examples/GilaOperators.jl:28-55,
+ * test/runtests.jl:38-49) -- a format only Julia can read.  Little-endian:
+ * a 512-byte header (magic "GILAFUR1", version, element type, options,
+ * volumes, branch / device lengths, stored extents, symmetric axes, element
+ * count, FNV-1a-64 of the payload), then the eight branches as
+ * gla_opr_export_fur returns them.  Loading checks the layout against the
+ * volumes, the size and the checksum; dtype 0 keeps the stored type,
+ * GLA_C64 / GLA_C128 converts. */
+gla_status gla_opr_save(gla_opr_t h, const char* path);
+gla_status gla_opr_load(const char* path, int32_t dtype, int32_t device, gla_opr_t* out);
+
 /* Bytes of device work memory the handle holds for nrhs right-hand sides. */
 int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs);
 
