@@ -259,7 +259,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                                                    AxisMap a1, AxisMap a2, AxisMap a3,
                                                    const cpx<R>* __restrict__ phz,
                                                    const cpx<R>* __restrict__ tw,
-                                                   const int* __restrict__ order, int per_xcd)
+                                                   const int* __restrict__ order, int per_xcd, int nrhs)
 {
     constexpr int E = N / TP;
     // WL: every pencil lives inside one wave (64/TP columns per wave, each
@@ -291,15 +291,22 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
     // share stored G^ entries (mirror images on symmetric axes) next to each
     // other in l, and l is chosen so that consecutive l run on one XCD
     // (workgroups are dealt round-robin over the 8 XCDs)
+    // With several right-hand sides the right-hand side is the fastest index
+    // of an XCD's stretch (grid 8 * per_xcd * nrhs): consecutive workgroups
+    // on one XCD run the same tile for every right-hand side, so its G^
+    // entries come from HBM once per launch rather than once per right-hand
+    // side.
     const int h = blockIdx.x;
-    const int l = (h % 8) * per_xcd + h / 8;
+    const int q = h / 8;
+    const int l = (h % 8) * per_xcd + q / nrhs;
+    const int rhs = q % nrhs;
     const int tile = order[l];
     if (tile < 0) return;   // padding entry (whole workgroup)
     const int m1 = (tile & 0xffff) * PX + xl;
     const int m2 = tile >> 16;
     const bool act = m1 < L1;
     const int64_t cs = (int64_t)Z * L2 * L1;      // component stride
-    cpx<R>* col = W2 + (int64_t)blockIdx.z * w2s + (int64_t)m2 * L1 + m1;
+    cpx<R>* col = W2 + (int64_t)rhs * w2s + (int64_t)m2 * L1 + m1;
     const int64_t zs = (int64_t)L2 * L1;
 
     // RELOAD: read the column again for the odd parity (an L2 hit) instead of
@@ -456,9 +463,10 @@ __global__ __launch_bounds__(TP * PX, 2) void k_zpipe(cpx<R>* __restrict__ W2, i
                                                    AxisMap a1, AxisMap a2, AxisMap a3,
                                                    const cpx<R>* __restrict__ phz,
                                                    const cpx<R>* __restrict__ tw,
-                                                   const int* __restrict__ order, int per_xcd)
+                                                   const int* __restrict__ order, int per_xcd, int nrhs)
 {
     static_assert(MODE != 1, "the pipelined pass needs S3 <= B3");
+    (void)nrhs;   // right-hand side from blockIdx.z in this form
     constexpr int E = N / TP;
     __shared__ cpx<R> lds[KB * PX * N];
     int xl = threadIdx.x % PX;

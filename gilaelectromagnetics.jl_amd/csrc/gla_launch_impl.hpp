@@ -24,6 +24,13 @@
 #ifndef GLA_ZBATCH_KB
 #define GLA_ZBATCH_KB 64
 #endif
+// axis-2 passes: consecutive axis-1 columns per workgroup (128-byte rows)
+#ifndef GLA_PXY_F
+#define GLA_PXY_F 16
+#endif
+#ifndef GLA_PXY_D
+#define GLA_PXY_D 8
+#endif
 #ifndef GLA_ZWAVE
 #define GLA_ZWAVE 0
 #endif
@@ -83,7 +90,7 @@ __host__ __device__ constexpr int pick_e(int N, int target, int maxtp, bool wave
 template <typename R>
 struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
-    static constexpr int PX = DBL ? 8 : 16;           // 128-byte rows across a bundle
+    static constexpr int PX = DBL ? GLA_PXY_D : GLA_PXY_F;   // axis-2 passes: columns per bundle
     static constexpr int E_ROW = GLA_EROW;            // passes 1, 2, 4, 5
     static constexpr int E_Z = DBL ? GLA_EZ_D : GLA_EZ_F;   // pass 3 (3 components live)
     static constexpr int PXZ = zpass_px(sizeof(cpx<R>));       // pass-3 columns per workgroup
@@ -158,18 +165,18 @@ static hipError_t launch_inv_y(const Geom& g, const DevTab& tb, const void* W2, 
 }
 
 template <typename R, int N>
-static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
+static hipError_t launch_zpass_chunk(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
 {
     using K = LenK<R, N>;
     constexpr int PX = Cfg<R>::PXZ;
     constexpr int SC = Cfg<R>::ZSCHED;
     constexpr int PF = sizeof(R) == 8 ? GLA_ZPF_D : GLA_ZPF_F;
-    dim3 grid(8 * tb.zper_xcd, 1, nrhs);
+    dim3 grid(8 * tb.zper_xcd * nrhs, 1, 1);   // right-hand side inside the XCD stretch
     dim3 blk(K::TP_Z * PX);
     const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);
 #define GLA_ZARGS                                                                                   \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
-        tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd
+        tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd, nrhs
     if (GLA_ZPIPE && mode != 1 && !K::WL_Z) {
         // one resident wave of workgroups: blocks per CU from the occupancy
         // of this kernel, 8 XCDs of CUs/8 each
@@ -194,6 +201,22 @@ static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nr
     else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
 #undef GLA_ZARGS
     return hipGetLastError();
+}
+
+// pass 3 over nrhs right-hand sides: one launch has 8 * per_xcd workgroups per
+// right-hand side, so large batches are split to keep the grid within 2^31 - 1
+template <typename R, int N>
+static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
+{
+    const int64_t per = 8LL * tb.zper_xcd;
+    const int64_t lim = 2147483647LL / (per > 0 ? per : 1);
+    const int maxr = (int)(lim < 1 ? 1 : (lim < nrhs ? lim : nrhs));
+    for (int r0 = 0; r0 < nrhs; r0 += maxr) {
+        const int cnt = nrhs - r0 < maxr ? nrhs - r0 : maxr;
+        const hipError_t e = launch_zpass_chunk<R, N>(g, tb, (cpx<R>*)W2 + (int64_t)r0 * g.w2s, cnt, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <typename R, int N>
