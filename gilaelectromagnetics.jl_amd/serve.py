@@ -16,10 +16,12 @@ arrive over time and are solved together on one GPU:
   order when a slot is free or a new slab fits the memory budget; the budget
   counts every byte of a slab and of the operators (Green data and the
   library's per-RHS work buffers).
-* Rounds: every round runs one Arnoldi step for all active requests of a grid
-  in one batched Lippmann-Schwinger apply (gla_ls_apply, one medium per
-  right-hand side), with the grids on separate streams so their kernels
-  overlap, and one host synchronisation for all grids.  A request joins at
+* Steps: one Arnoldi step covers all active requests of a grid in one
+  batched Lippmann-Schwinger apply (gla_ls_apply, one medium per right-hand
+  side).  Every grid has its own stream and at most one step or refinement
+  in flight; the host waits for the grid whose work was queued first, does
+  its rotations and bookkeeping and queues its next step while the other
+  grids' kernels run.  A request joins at
   any round: the Gram-Schmidt products over a slab use the columns up to the
   largest step in the batch, and a request's columns beyond its own step are
   zero.
@@ -160,6 +162,9 @@ class _Grid:
         self.last_used = time.perf_counter()
         self.pending = None
         self.reorth = None
+        self.state = None           # None, "step", "reorth" or "refine": what is in flight
+        self.seq = 0
+        self.event = torch.cuda.Event() if self.stream is not None else None
 
     @staticmethod
     def slot_bytes(n, ncell, restart, inner):
@@ -253,28 +258,29 @@ class _Grid:
                 h2, _, n1 = gs_pass(self.kry, self.V, k, jm + 1, self.W, st)
                 h = h + h2
             host = [t.to("cpu", non_blocking=True) for t in (h, n0, n1)]
-            self.pending = (host, jt, ar, jm)
+            self.pending = (host, jt, ar, jm, k)
+            self.record()
 
     def launch_reorth(self):
         """The second Gram-Schmidt pass when gs is "dgks" and a slot lost more
         than 1 - 1/sqrt(2) of its norm (cgs2 ran it in launch_step already).
         Returns whether one was launched."""
-        (h, n0, n1), jt, ar, jm = self.pending
+        (h, n0, n1), jt, ar, jm, k = self.pending
         self.reorth = None
         if self.gs != "dgks" or not (np.sqrt(n1.numpy()) < DGKS * np.sqrt(n0.numpy())).any():
             return False
         with self.ctx():
-            h2, _, n2 = gs_pass(self.kry, self.V, self.k, jm + 1, self.W, self.stream_handle())
+            h2, _, n2 = gs_pass(self.kry, self.V, k, jm + 1, self.W, self.stream_handle())
             self.reorth = (h2.to("cpu", non_blocking=True), n2.to("cpu", non_blocking=True))
+            self.record()
         return True
 
     def finish_step(self):
         """Host part of the step: the next basis vector, Givens rotations and
         residual estimates.  Returns the slots whose cycle ends."""
         torch = _torch()
-        (h, n0, n1), jt, ar, jm = self.pending
+        (h, n0, n1), jt, ar, jm, k = self.pending
         self.pending = None
-        k = self.k
         m = self.m
         hcol = h.numpy()
         hn = np.sqrt(n1.numpy())
@@ -385,6 +391,17 @@ class _Grid:
             self.r[idx] = self.b[idx] - self.t64[:nb]
             rn = torch.linalg.vector_norm(self.r[idx], dim=1)
             self.pending_refine = (slots, rn.to("cpu", non_blocking=True))
+            self.record()
+
+    def record(self):
+        """Mark the end of the work just queued on this grid's stream."""
+        if self.stream is not None:
+            self.event.record(self.stream)
+
+    def wait(self):
+        """Block until this grid's queued work is done (other streams run on)."""
+        if self.stream is not None:
+            self.event.synchronize()
 
 
 class Server:
@@ -427,6 +444,7 @@ class Server:
         self.used = 0
         self.rounds = 0
         self.reorth_rounds = 0
+        self._seq = 0
         # host seconds per phase of the rounds, and slot-steps run
         self.times = dict.fromkeys(("launch", "wait_step", "reorth", "finish", "refine_launch", "wait_refine",
                                     "complete", "slot_rounds"), 0.0)
@@ -442,7 +460,7 @@ class Server:
         for key, g in sorted(self.grids.items(), key=lambda kv: kv[1].last_used):
             if self.used + need <= self.mem_budget:
                 return
-            if g.k == 0:
+            if g.k == 0 and g.state is None:
                 self.used -= g.bytes
                 del self.grids[key]
         busy = set(self.grids)
@@ -459,7 +477,7 @@ class Server:
         key = req.key
         g = self.grids.get(key)
         if g is not None:
-            return g if g.k < g.cap else None
+            return g if (g.k < g.cap and g.state is None) else None
         if not allow_new:
             return None
         ops = self.ops.get(key)
@@ -528,65 +546,69 @@ class Server:
         return sum(g.k for g in self.grids.values())
 
     def round(self):
-        """Admit what fits, then one Arnoldi step for every active request."""
+        """One scheduler iteration: admit what fits, queue an Arnoldi step on
+        every idle grid with requests (each on its own stream), then wait for
+        the grid whose work was queued first and do its host part -- the
+        other grids' kernels keep the GPU busy meanwhile.  Returns the number
+        of grids processed (0 when nothing is in flight)."""
         torch = _torch()
-        self._admit(time.perf_counter())
-        grids = [g for g in self.grids.values() if g.k > 0]
-        if not grids:
-            return 0
         tm = self.times
         t0 = time.perf_counter()
-        for g in grids:
-            g.last_used = t0
-            g.launch_step()
+        self._admit(t0)
+        for g in self.grids.values():
+            if g.state is None and g.k > 0:
+                g.last_used = t0
+                g.launch_step()
+                g.state = "step"
+                self._seq += 1
+                g.seq = self._seq
+                tm["slot_rounds"] += g.k
+                self.rounds += 1
+        busy = [g for g in self.grids.values() if g.state is not None]
+        if not busy:
+            return 0
+        g = min(busy, key=lambda x: x.seq)
         t1 = time.perf_counter()
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-        t2 = time.perf_counter()
         tm["launch"] += t1 - t0
-        tm["wait_step"] += t2 - t1
-        tm["slot_rounds"] += sum(g.k for g in grids)
-        self.rounds += 1
-        if any([g.launch_reorth() for g in grids]):
+        g.wait()
+        t2 = time.perf_counter()
+        tm["wait_step" if g.state != "refine" else "wait_refine"] += t2 - t1
+        if g.state == "step" and g.launch_reorth():
             self.reorth_rounds += 1
-            if torch.cuda.is_available():
-                torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        ends = {}
-        for g in grids:
-            ends[id(g)] = g.finish_step()
-        refine = [(g, ends[id(g)]) for g in grids if ends[id(g)]]
-        t4 = time.perf_counter()
-        for g, slots in refine:
-            g.launch_refine(slots)
-        t5 = time.perf_counter()
-        if refine and torch.cuda.is_available():
-            torch.cuda.synchronize()
-        now = time.perf_counter()
-        tm["reorth"] += t3 - t2
-        tm["finish"] += t4 - t3
-        tm["refine_launch"] += t5 - t4
-        tm["wait_refine"] += now - t5
-        for g, slots in refine:
+            g.state = "reorth"
+            self._seq += 1
+            g.seq = self._seq
+        elif g.state in ("step", "reorth"):
+            ends = g.finish_step()
+            if ends:
+                g.launch_refine(ends)
+                g.state = "refine"
+                self._seq += 1
+                g.seq = self._seq
+            else:
+                g.state = None
+            tm["finish"] += time.perf_counter() - t2
+        else:
             slots, rn = g.pending_refine
             rn = rn.numpy()
             finished = []
             for s, v in zip(slots, rn):
                 g.rn[s] = float(v)
                 req = g.reqs[s]
-                rel = g.rn[s] / g.bn[s]
-                if rel <= req.tol:
+                if g.rn[s] / g.bn[s] <= req.tol:
                     finished.append((s, True))
                 elif req.cycles >= self.max_cycles:
                     finished.append((s, False))
                 else:
                     with g.ctx():
                         g.start_cycle(s)
+            g.state = None
             # remove from the highest slot down so compaction keeps indices valid
+            now = time.perf_counter()
             for s, conv in sorted(finished, reverse=True):
                 self._complete(g, s, now, conv)
-        tm["complete"] += time.perf_counter() - now
-        return len(grids)
+            tm["complete"] += time.perf_counter() - t2
+        return 1
 
     def run(self, requests, arrivals=None, timeout=None, on_done=None):
         """Serve `requests`; arrivals[i] (seconds from the start) is when
