@@ -31,6 +31,9 @@
 #ifndef GLA_PXY_D
 #define GLA_PXY_D 8
 #endif
+#ifndef GLA_YLDS_KB
+#define GLA_YLDS_KB 160
+#endif
 #ifndef GLA_ZWAVE
 #define GLA_ZWAVE 0
 #endif
@@ -104,7 +107,14 @@ struct LenK {
     using C = Cfg<R>;
     static constexpr int TP_ROW = N / pick_e(N, C::E_ROW, 64, true);
     static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
-    static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / C::PX);
+    // axis-2 passes: the bundle width, capped so the two parities' LDS
+    // buffers stay within GLA_YLDS_KB
+    static constexpr int PXY = [] {
+        int px = C::PX;
+        while (px > 1 && 2 * px * N * (int)sizeof(cpx<R>) > GLA_YLDS_KB * 1024) px /= 2;
+        return px;
+    }();
+    static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / PXY);
     static constexpr int TP_Z = N / pick_e(N, C::E_Z, 1024 / C::PXZ, GLA_ZWAVE != 0);
     // pass 3 keeps each pencil inside one wave when its threads divide 64
     static constexpr int WL_Z = (GLA_ZWAVE != 0 && 64 % TP_Z == 0 && (TP_Z * C::PXZ) % 64 == 0) ? 1 : 0;
@@ -144,7 +154,7 @@ template <typename R, int N>
 static hipError_t launch_fwd_y(const Geom& g, const DevTab& tb, const void* W1, void* W2, int nrhs, hipStream_t st)
 {
     using K = LenK<R, N>;
-    constexpr int PX = Cfg<R>::PX;
+    constexpr int PX = K::PXY;
     dim3 grid((g.L[0] + PX - 1) / PX, g.S[2] * 3, nrhs);
     hipLaunchKernelGGL((k_fwd_y<R, N, K::TP_Y, PX, 1>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W1,
                        (cpx<R>*)W2, g.L[0], g.S[1], g.S[2], g.Y1, g.Z, g.w1s, g.w2s,
@@ -156,7 +166,7 @@ template <typename R, int N>
 static hipError_t launch_inv_y(const Geom& g, const DevTab& tb, const void* W2, void* W1, int nrhs, hipStream_t st)
 {
     using K = LenK<R, N>;
-    constexpr int PX = Cfg<R>::PX;
+    constexpr int PX = K::PXY;
     dim3 grid((g.L[0] + PX - 1) / PX, g.T[2] * 3, nrhs);
     hipLaunchKernelGGL((k_inv_y<R, N, K::TP_Y, PX, 1>), grid, dim3(K::TP_Y * PX), 0, st, (const cpx<R>*)W2,
                        (cpx<R>*)W1, g.L[0], g.T[1], g.T[2], g.Y1, g.Z, g.w2s, g.w1s,
@@ -223,7 +233,7 @@ template <typename R, int N>
 static hipError_t launch_fwd_z(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
 {
     using K = LenK<R, N>;
-    constexpr int PX = Cfg<R>::PX;
+    constexpr int PX = K::PXY;
     dim3 grid((g.L[0] + PX - 1) / PX, g.L[1] * 3, nrhs);
     hipLaunchKernelGGL((k_fwd_z<R, N, K::TP_Y, PX, 1>), grid, dim3(K::TP_Y * PX), 0, st, (cpx<R>*)W2, g.L[0], g.L[1],
                        g.S[2], g.Z, g.w2s, (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2]);

@@ -35,7 +35,7 @@ def _mem_self(gila, orc, N, typ, model=0):
 
 
 SELF_SHAPES = [(1, 1, 1), (2, 2, 2), (4, 4, 4), (3, 5, 6), (16, 16, 16), (8, 12, 16), (10, 20, 40), (16, 32, 16),
-               (24, 48, 64), (80, 40, 40), (96, 8, 6)]
+               (24, 48, 64), (80, 40, 40), (96, 8, 6), (24, 24, 24), (32, 48, 64), (64, 64, 64)]
 
 
 @pytest.mark.gpu
@@ -208,3 +208,26 @@ def test_profile_entry_leaves_a_valid_matvec(gila, orc, gpu):
     assert all(ms[i] > 0 for i in range(5)) and all(by[i] > 0 for i in range(5))
     y = yd.cpu().numpy().reshape(N + (3,), order="F")
     assert orc.rel_l2(y, orc.matvec(N, N, (1, 1, 1), fur, x)) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_batched_96_cubed_plane_waves_and_dipoles(gila, orc, gpu):
+    # the 96^3 batched workload's right-hand sides (plane waves with random
+    # directions and polarisations, point dipoles), 4 of them in one launch,
+    # each column against the FP64 oracle
+    from fractions import Fraction
+    from gila_amd import inputs
+    N = (96, 96, 96)
+    print("case batched 96^3 plane waves + dipoles", flush=True)
+    mem, fur = _mem_self(gila, orc, N, np.complex64)
+    cols = []
+    for i in range(2):
+        khat = inputs.random_unit(7000 + i)
+        p = inputs.random_unit(8000 + i)
+        p = p - khat * (p @ khat)
+        cols.append(inputs.plane_wave(N, (Fraction(1, 32),) * 3, khat, p / np.linalg.norm(p)))
+        cols.append(inputs.point_dipole(N, [11 + 37 * i, 50, 95 - 40 * i], inputs.random_unit(9100 + i)))
+    x = np.stack(cols, axis=4)
+    y = _dev_apply(gila, mem, x, nrhs=4)
+    for r in range(4):
+        assert orc.rel_l2(y[..., r], orc.matvec(N, N, (1, 1, 1), fur, x[..., r])) <= 1e-5
