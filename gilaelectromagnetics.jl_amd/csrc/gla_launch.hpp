@@ -1,5 +1,6 @@
 // Launch layer between the C-ABI (gla_capi.cpp) and the pass kernels.
 #pragma once
+#include "gla_tune.hpp"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -31,17 +32,9 @@ struct DevTab {
     int zper_xcd;          // pass-3 workgroups per XCD (grid = 8 * zper_xcd)
 };
 
-// columns per pass-3 workgroup (overridable at compile time for variant timing)
-#ifndef GLA_PXZ_F
-#define GLA_PXZ_F 16
-#endif
-#ifndef GLA_PXZ_D
-#define GLA_PXZ_D 8
-#endif
-constexpr int zpass_px(int complex_bytes) { return complex_bytes == 16 ? GLA_PXZ_D : GLA_PXZ_F; }
-
-// columns per pass-3 workgroup for the element type (bytes per complex)
-int zpass_tile_width(int complex_bytes);
+// columns per pass-3 workgroup for the element type (bytes per complex) and
+// the pass-3 transform length (the table of gla_tune.hpp)
+int zpass_tile_width(int complex_bytes, int n);
 
 // Pass-3 tile order: ntile = ceil(L1 / PX) column tiles times L2 rows,
 // padded to a multiple of 8 with -1.  With symmetric storage on axes 1 and 2

@@ -7,12 +7,6 @@
 #ifndef GLA_EROW
 #define GLA_EROW 8
 #endif
-#ifndef GLA_EZ_F
-#define GLA_EZ_F 8
-#endif
-#ifndef GLA_EZ_D
-#define GLA_EZ_D 4
-#endif
 // pass-3 schedule: 0 keeps the input column in registers through the even
 // parity, 2 splits both parities first (fewer registers for FP64)
 #ifndef GLA_ZRELOAD_F
@@ -95,12 +89,9 @@ struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
     static constexpr int PX = DBL ? GLA_PXY_D : GLA_PXY_F;   // axis-2 passes: columns per bundle
     static constexpr int E_ROW = GLA_EROW;            // passes 1, 2, 4, 5
-    static constexpr int E_Z = DBL ? GLA_EZ_D : GLA_EZ_F;   // pass 3 (3 components live)
-    static constexpr int PXZ = zpass_px(sizeof(cpx<R>));       // pass-3 columns per workgroup
     static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
 };
 
-static_assert(Cfg<float>::PXZ == zpass_px(8) && Cfg<double>::PXZ == zpass_px(16), "one pass-3 tile width");
 
 template <typename R, int N>
 struct LenK {
@@ -115,13 +106,15 @@ struct LenK {
         return px;
     }();
     static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / PXY);
-    static constexpr int TP_Z = N / pick_e(N, C::E_Z, 1024 / C::PXZ, GLA_ZWAVE != 0);
+    // pass 3: shape from the tuning table (gla_tune.hpp)
+    static constexpr int PXZ = ztune(sizeof(cpx<R>), N).px;
+    static constexpr int TP_Z = N / pick_e(N, ztune(sizeof(cpx<R>), N).e, 1024 / PXZ, GLA_ZWAVE != 0);
     // pass 3 keeps each pencil inside one wave when its threads divide 64
-    static constexpr int WL_Z = (GLA_ZWAVE != 0 && 64 % TP_Z == 0 && (TP_Z * C::PXZ) % 64 == 0) ? 1 : 0;
+    static constexpr int WL_Z = (GLA_ZWAVE != 0 && 64 % TP_Z == 0 && (TP_Z * PXZ) % 64 == 0) ? 1 : 0;
     static constexpr int PS_Z = WL_Z ? N + (N >> GLA_ZPADSH) + 1 : N;
     // pass 3 runs its three component transforms together when their LDS
     // buffers fit the budget
-    static constexpr int KB_Z = 3 * C::PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024 ? 3 : 1;
+    static constexpr int KB_Z = 3 * PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024 ? 3 : 1;
 };
 
 template <typename R, int N>
@@ -178,7 +171,7 @@ template <typename R, int N>
 static hipError_t launch_zpass_chunk(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
 {
     using K = LenK<R, N>;
-    constexpr int PX = Cfg<R>::PXZ;
+    constexpr int PX = K::PXZ;
     constexpr int SC = Cfg<R>::ZSCHED;
     constexpr int PF = sizeof(R) == 8 ? GLA_ZPF_D : GLA_ZPF_F;
     dim3 grid(8 * tb.zper_xcd * nrhs, 1, 1);   // right-hand side inside the XCD stretch

@@ -231,3 +231,19 @@ def test_batched_96_cubed_plane_waves_and_dipoles(gila, orc, gpu):
     y = _dev_apply(gila, mem, x, nrhs=4)
     for r in range(4):
         assert orc.rel_l2(y[..., r], orc.matvec(N, N, (1, 1, 1), fur, x[..., r])) <= 1e-5
+
+
+Z_LENGTHS = [16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", Z_LENGTHS)
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_every_tuned_pass3_length(gila, orc, gpu, n, typ):
+    # pass 3 runs a length-n transform with the shape the tuning table picks
+    # for (element type, n); every compiled length >= 16 against the oracle
+    N = (4, 6, n)
+    print(f"case pass-3 length {n} {np.dtype(typ).name}", flush=True)
+    mem, fur = _mem_self(gila, orc, N, typ)
+    x = orc.complex_gaussian(N + (3,), seed=n)
+    assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(N, N, (1, 1, 1), fur, x)) <= TOL[typ]
