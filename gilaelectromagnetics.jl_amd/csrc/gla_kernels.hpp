@@ -115,6 +115,98 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
 }
 
 // --------------------------------------------------------------------------
+// Passes 1 and 2 fused (B1 = B2 = N): one workgroup per (plane (i3, c), x
+// parity p1, right-hand side).  Phase 1 transforms the plane's rows (x split
+// for parity p1, length-N FFT) into an LDS tile, row i2 at tile[i2*RS ..];
+// each row pencil uses its own tile row as exchange space.  Phase 2 reads the
+// tile's columns (y split into both parities, two length-N FFTs, each column
+// pencil using its own column slots as exchange space) and writes W2
+// directly: the intermediate W1 of the two-pass form never goes to HBM.
+// Rows beyond S2 are scratch for masked pencils; the tile holds
+// XY_ROWS(N) >= max(S2 rounded to the row-pencil count, N) rows.
+// --------------------------------------------------------------------------
+template <int N>
+struct XyTile {
+    static constexpr int RS = (N + (N >> 3) + 1) | 1;   // odd pitch: column accesses spread over banks
+    static constexpr int SLOTS = 20480;                  // 160 KB of ComplexF32
+    static constexpr int ROWS = SLOTS / RS;
+};
+
+template <typename R, int N, int TP1, int TP2, int NT>
+__global__ __launch_bounds__(NT, 1) void k_fwd_xy(const cpx<R>* __restrict__ X, cpx<R>* __restrict__ W2, int S1,
+                                                  int S2, int S3, int L1, int Z, int64_t xs, int64_t w2s,
+                                                  const cpx<R>* __restrict__ phz1, const cpx<R>* __restrict__ tw1,
+                                                  const cpx<R>* __restrict__ phz2, const cpx<R>* __restrict__ tw2)
+{
+    using T = XyTile<N>;
+    constexpr int RS = T::RS;
+    constexpr int E1 = N / TP1, E2 = N / TP2;
+    constexpr int PEN1 = NT / TP1, CPB = NT / TP2;
+    static_assert(64 % TP1 == 0 && 64 % TP2 == 0 && NT % 64 == 0, "pencils inside one wave");
+    static_assert(N % CPB == 0, "whole column groups");
+    static_assert(T::ROWS >= N, "the tile holds N rows");
+    __shared__ cpx<R> tile[T::SLOTS];
+    const int pl = blockIdx.x;
+    const int c = pl / S3, i3 = pl - c * S3;
+    const int p1 = blockIdx.y;
+    const int64_t rhs = blockIdx.z;
+
+    // phase 1: rows i2 of the plane, x split for parity p1 and x-FFT
+    {
+        const int t = threadIdx.x % TP1, pen = threadIdx.x / TP1;
+        const cpx<R>* xp = X + rhs * xs + ((int64_t)c * S3 + i3) * S2 * (int64_t)S1;
+        for (int r0 = 0; r0 < S2; r0 += PEN1) {
+            const int i2 = r0 + pen;
+            const bool act = i2 < S2;
+            const cpx<R>* xr = xp + (int64_t)(act ? i2 : 0) * S1;
+            cpx<R> v[1][E1];
+#pragma unroll
+            for (int e = 0; e < E1; ++e) {
+                const int n = t + TP1 * e;
+                cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
+                if (act && n < S1) a = xr[n];
+                if (act && n + N < S1) b = xr[n + N];
+                v[0][e] = p1 == 0 ? a + b : phz1[n] * (a - b);
+            }
+            PencilFft<R, N, TP1, 1, RS, false, 1, 1, 0, 3>::run(v, tile, t, i2, tw1);
+#pragma unroll
+            for (int e = 0; e < E1; ++e) tile[i2 * RS + t + TP1 * e] = v[0][e];
+        }
+    }
+    __syncthreads();
+
+    // phase 2: columns k1 of the tile, y split into both parities and y-FFTs
+    {
+        const int t = threadIdx.x % TP2, pen = threadIdx.x / TP2;
+        cpx<R>* dst = W2 + rhs * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + (int64_t)p1 * N;
+        for (int k0 = 0; k0 < N; k0 += CPB) {
+            const int k1 = k0 + pen;
+            cpx<R> a[E2], b[E2];
+#pragma unroll
+            for (int e = 0; e < E2; ++e) {
+                const int n = t + TP2 * e;
+                a[e] = n < S2 ? tile[n * RS + k1] : cpx<R>{R(0), R(0)};
+                b[e] = n + N < S2 ? tile[(n + N) * RS + k1] : cpx<R>{R(0), R(0)};
+            }
+            lds_sync<1>();   // the column's reads are done before its slots become exchange space
+#pragma unroll
+            for (int p2 = 0; p2 < 2; ++p2) {
+                cpx<R> v[1][E2];
+#pragma unroll
+                for (int e = 0; e < E2; ++e) {
+                    const int n = t + TP2 * e;
+                    v[0][e] = p2 == 0 ? a[e] + b[e] : phz2[n] * (a[e] - b[e]);
+                }
+                PencilFft<R, N, TP2, RS, 1, false, 1, 1, 0, 0>::run(v, tile, t, k1, tw2);
+#pragma unroll
+                for (int e = 0; e < E2; ++e)
+                    dst[(int64_t)(p2 * N + t + TP2 * e) * L1 + k1] = v[0][e];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
 // Pass 5: x-inverse + merge/crop onto T1, optional y = alpha*res + beta*y.
 // --------------------------------------------------------------------------
 template <typename R, int N, int TP, int PEN, int MINW>

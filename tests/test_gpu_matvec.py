@@ -247,3 +247,41 @@ def test_every_tuned_pass3_length(gila, orc, gpu, n, typ):
     mem, fur = _mem_self(gila, orc, N, typ)
     x = orc.complex_gaussian(N + (3,), seed=n)
     assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(N, N, (1, 1, 1), fur, x)) <= TOL[typ]
+
+
+FUSED_SELF = [(32, 32, 8), (64, 64, 6), (96, 96, 4), (128, 128, 2)]
+FUSED_EXT = [((32, 40, 4), (32, 24, 4)), ((32, 24, 4), (32, 40, 4)), ((64, 70, 3), (64, 58, 5))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", FUSED_SELF)
+def test_fused_xy_self(gila, orc, gpu, N):
+    # ComplexF32 with B1 = B2 in {32, 64, 96, 128}: the shapes the fused
+    # pass-1+2 kernel covers when the library is built with it (GLA_XYFUSE=1;
+    # the default build runs them through the two passes)
+    print(f"case fused xy self N={N}", flush=True)
+    mem, fur = _mem_self(gila, orc, N, np.complex64)
+    x = orc.complex_gaussian(N + (3, 2), seed=70)
+    y = _dev_apply(gila, mem, x, nrhs=2)
+    for r in range(2):
+        assert orc.rel_l2(y[..., r], orc.matvec(N, N, (1, 1, 1), fur, x[..., r])) <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("S,T", FUSED_EXT)
+def test_fused_xy_external_spill_and_zero_rows(gila, orc, gpu, S, T):
+    # S2 > B2 (spill-over rows added) and S2 < B2 (zero rows) at fused-kernel
+    # shapes (B1 = B2)
+    print(f"case fused xy ext S={S} T={T}", flush=True)
+    delta = (0.25, -0.5, 0.75)
+    crc = orc.crc_ext(S, T, (H, H, H), delta)
+    sym = (0, 0, 0)
+    fur = orc.fur_from_crc(S, T, sym, crc)
+    B, U = orc.dims(S, T, sym)
+    assert B[0] == B[1]
+    src = gila.GlaVol(S, ((1, 32),) * 3, (0, 0, 0))
+    trg = gila.GlaVol(T, ((1, 32),) * 3, ((1, 4), (1, 2), (3, 4)))
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), trg, src, egoFur=orc.fur_branches(fur.astype(np.complex64), U),
+                         setTyp=np.complex64)
+    x = orc.complex_gaussian(tuple(S) + (3,), seed=71)
+    assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(S, T, sym, fur, x)) <= 1e-5
