@@ -764,7 +764,7 @@ static gla_status krylov_check(int32_t dtype, int64_t k, int64_t nv, int64_t n)
 
 gla_status gla_krylov_dots(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* w, int64_t w_rhs_stride, void* part,
-                           int64_t part_bytes, void* h, double* w2, void* stream)
+                           int64_t part_bytes, void* h, double* w2, const int64_t* ncol, void* stream)
 {
     if (gla_status s = krylov_check(dtype, k, nv, n)) return s;
     if (k == 0 || (nv == 0 && !w2)) return GLA_OK;
@@ -774,16 +774,17 @@ gla_status gla_krylov_dots(int32_t dtype, int64_t k, int64_t nv, int64_t n, cons
         return fail(GLA_ERR_SIZE, "partial-sum workspace smaller than gla_krylov_dots_workspace()");
     const hipStream_t st = (hipStream_t)stream;
     const hipError_t e = dtype == GLA_C64
-        ? gla::run_krylov_dots<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, w2, st)
+        ? gla::run_krylov_dots<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, w2,
+                                      ncol, st)
         : gla::run_krylov_dots<double>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, w, w_rhs_stride, part, h, w2,
-                                       st);
+                                       ncol, st);
     return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
 }
 
 gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* h, int64_t h_rhs_stride, void* w,
                            int64_t w_rhs_stride, double s, void* part, int64_t part_bytes, double* w2,
-                           void* stream)
+                           const int64_t* ncol, void* stream)
 {
     if (gla_status st = krylov_check(dtype, k, nv, n)) return st;
     if (k == 0 || n == 0) return GLA_OK;
@@ -793,9 +794,9 @@ gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, cons
     const hipStream_t hs = (hipStream_t)stream;
     const hipError_t e = dtype == GLA_C64
         ? gla::run_krylov_axpy<float>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, h, h_rhs_stride, w,
-                                      w_rhs_stride, s, part, w2, hs)
+                                      w_rhs_stride, s, part, w2, ncol, hs)
         : gla::run_krylov_axpy<double>(k, (int)nv, n, V, v_rhs_stride, v_vec_stride, h, h_rhs_stride, w,
-                                       w_rhs_stride, s, part, w2, hs);
+                                       w_rhs_stride, s, part, w2, ncol, hs);
     return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
 }
 
@@ -809,6 +810,22 @@ gla_status gla_krylov_rows(int32_t dtype, int64_t k, int64_t n, void* V, int64_t
     const hipError_t e = dtype == GLA_C64
         ? gla::run_krylov_rows<float>(k, n, V, v_rhs_stride, v_vec_stride, idx, b, b_rhs_stride, scale, hs)
         : gla::run_krylov_rows<double>(k, n, V, v_rhs_stride, v_vec_stride, idx, b, b_rhs_stride, scale, hs);
+    return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
+}
+
+gla_status gla_krylov_next(int32_t dtype, int64_t k, int64_t n, void* V, int64_t v_rhs_stride, int64_t v_vec_stride,
+                           const int64_t* idx, const void* w, int64_t w_rhs_stride, const double* w2, void* stage,
+                           int64_t stage_rhs_stride, void* stream)
+{
+    if (gla_status st = krylov_check(dtype, k, 0, n)) return st;
+    if (k == 0 || n == 0) return GLA_OK;
+    if (!V || !idx || !w || !w2) return fail(GLA_ERR_ARG, "null argument");
+    const hipStream_t hs = (hipStream_t)stream;
+    const hipError_t e = dtype == GLA_C64
+        ? gla::run_krylov_next<float>(k, n, V, v_rhs_stride, v_vec_stride, idx, w, w_rhs_stride, w2, stage,
+                                      stage_rhs_stride, hs)
+        : gla::run_krylov_next<double>(k, n, V, v_rhs_stride, v_vec_stride, idx, w, w_rhs_stride, w2, stage,
+                                       stage_rhs_stride, hs);
     return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
 }
 

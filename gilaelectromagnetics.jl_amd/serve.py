@@ -22,9 +22,10 @@ arrive over time and are solved together on one GPU:
   in flight; the host waits for the grid whose work was queued first, does
   its rotations and bookkeeping and queues its next step while the other
   grids' kernels run.  A request joins at
-  any round: the Gram-Schmidt products over a slab use the columns up to the
-  largest step in the batch, and a request's columns beyond its own step are
-  zero.
+  any round: in one batched step each request's Gram-Schmidt products run
+  over its own basis columns only (its own step count), and its next basis
+  vector is normalized on the device from the norm the update leaves there,
+  so the only host round trip of a step is the Hessenberg column.
 * Precision: GMRES(restart) cycles in FP32 on the residual equation; at the
   end of each cycle the update and the residual b - A x are formed in FP64
   (iterative refinement) and the request completes when
@@ -165,6 +166,18 @@ class _Grid:
         self.state = None           # None, "step", "reorth" or "refine": what is in flight
         self.seq = 0
         self.event = torch.cuda.Event() if self.stream is not None else None
+        # device-resident step (one and two classical passes on the GPU): the
+        # next basis vector is normalized on the device from the norm the
+        # update kernel leaves there, and also written to Xin, the next
+        # step's matvec input; each slot's products run over its own columns
+        # 0..j only (stale columns past j are never read, so a restart need
+        # not clear them).  Xin[s] always holds slot s's current vector.
+        self.fused = self.V.is_cuda and gs != "dgks"
+        if self.fused:
+            self.jh = torch.empty((2, cap), dtype=torch.int64, pin_memory=True)
+            self.jd = torch.empty((2, cap), dtype=torch.int64, device=device)
+            self.gsout = torch.empty(cap * (2 * (m + 1) + 2), dtype=torch.float64, device=device)
+            self.gsout2 = torch.empty_like(self.gsout) if gs == "cgs2" else None
 
     @staticmethod
     def slot_bytes(n, ncell, restart, inner):
@@ -194,8 +207,12 @@ class _Grid:
     # --- slot management ---------------------------------------------------
     def start_cycle(self, s):
         """Arnoldi restart of slot s from its residual r (norm rn[s])."""
-        self.V[s, 1:].zero_()
-        self.V[s, 0].copy_((self.r[s] / self.rn[s]).to(self.cdt))
+        if self.fused:
+            self.V[s, 0].copy_((self.r[s] / self.rn[s]).to(self.cdt))
+            self.Xin[s].copy_(self.V[s, 0])
+        else:
+            self.V[s, 1:].zero_()
+            self.V[s, 0].copy_((self.r[s] / self.rn[s]).to(self.cdt))
         self.H[s] = 0
         self.g[s] = 0
         self.g[s, 0] = 1.0
@@ -228,7 +245,7 @@ class _Grid:
         last = self.k - 1
         if s != last:
             with self.ctx():
-                for t in (self.V, self.chi_in, self.chi64, self.x, self.b, self.r):
+                for t in (self.V, self.chi_in, self.chi64, self.x, self.b, self.r) + ((self.Xin,) if self.fused else ()):
                     t[s].copy_(t[last])
             for a in (self.H, self.cs, self.sn, self.g, self.j, self.rn, self.bn):
                 a[s] = a[last]
@@ -245,6 +262,9 @@ class _Grid:
         k = self.k
         jm = int(self.j[:k].max())
         st = self.stream_handle()
+        if self.fused:
+            self._launch_step_fused(k, jm, st)
+            return
         with self.ctx():
             jt = torch.from_numpy(self.j[:k].copy()).to(self.dev)
             ar = torch.arange(k, device=self.dev)
@@ -261,13 +281,36 @@ class _Grid:
             self.pending = (host, jt, ar, jm, k)
             self.record()
 
+    def _launch_step_fused(self, k, jm, st):
+        """launch_step with the basis products over each slot's own columns
+        and the next vector formed on the device (no gather, no host norm)."""
+        nv = jm + 1
+        jh = self.jh.numpy()
+        jh[0, :k] = self.j[:k]
+        jh[1, :k] = self.j[:k] + 1
+        with self.ctx():
+            self.jd.copy_(self.jh, non_blocking=True)
+            ncol = self.jd[1, :k]
+            self.apply_inner(self.Xin[:k], self.chi_in[:k], self.W[:k])
+            h, n0, n1 = gs_pass(self.kry, self.V, k, nv, self.W, st, ncol=ncol, out=self.gsout)
+            if self.gs == "cgs2":
+                h2, _, n1b = gs_pass(self.kry, self.V, k, nv, self.W, st, ncol=ncol, out=self.gsout2)
+                h.add_(h2)
+                n1.copy_(n1b)
+            self.kry.next(self.V, k, ncol, self.W, n1, st, stage=self.Xin)
+            host = self.gsout[:k * (2 * nv + 2)].to("cpu", non_blocking=True)
+            self.pending = ("fused", host, jm, k)
+            self.record()
+
     def launch_reorth(self):
         """The second Gram-Schmidt pass when gs is "dgks" and a slot lost more
         than 1 - 1/sqrt(2) of its norm (cgs2 ran it in launch_step already).
         Returns whether one was launched."""
-        (h, n0, n1), jt, ar, jm, k = self.pending
         self.reorth = None
-        if self.gs != "dgks" or not (np.sqrt(n1.numpy()) < DGKS * np.sqrt(n0.numpy())).any():
+        if self.gs != "dgks":
+            return False
+        (h, n0, n1), jt, ar, jm, k = self.pending
+        if not (np.sqrt(n1.numpy()) < DGKS * np.sqrt(n0.numpy())).any():
             return False
         with self.ctx():
             h2, _, n2 = gs_pass(self.kry, self.V, k, jm + 1, self.W, self.stream_handle())
@@ -279,9 +322,16 @@ class _Grid:
         """Host part of the step: the next basis vector, Givens rotations and
         residual estimates.  Returns the slots whose cycle ends."""
         torch = _torch()
+        if self.pending[0] == "fused":
+            _, host, jm, k = self.pending
+            self.pending = None
+            arr = host.numpy()
+            nv = jm + 1
+            hcol = arr[:2 * k * nv].view(np.complex128).reshape(k, nv)
+            hn = np.sqrt(arr[2 * k * nv + k:2 * k * nv + 2 * k])
+            return self._advance(hcol, hn, k, jm)
         (h, n0, n1), jt, ar, jm, k = self.pending
         self.pending = None
-        m = self.m
         hcol = h.numpy()
         hn = np.sqrt(n1.numpy())
         if self.reorth is not None:
@@ -295,6 +345,12 @@ class _Grid:
                 self.kry.rows(self.V, k, jt + 1, self.W, self.stream_handle(), scale=scale)
             else:
                 self.V[ar, jt + 1] = self.W[:k] / scale.to(self.W.real.dtype).unsqueeze(1)
+        return self._advance(hcol, hn, k, jm)
+
+    def _advance(self, hcol, hn, k, jm):
+        """Givens update with the step's coefficients, then the slots whose
+        cycle ends."""
+        m = self.m
         js = self.j[:k].copy()
         if k <= 4:
             self._rotate_scalar(hcol, hn, js, jm)

@@ -123,24 +123,34 @@ class KrylovOps:
     def _ptr(t):
         return None if t is None else ctypes.c_void_p(t.data_ptr())
 
-    def dots(self, V, k, nv, w, h, stream, w2=None):
+    def dots(self, V, k, nv, w, h, stream, w2=None, ncol=None):
         """h[r, i] = <V[r, i], w[r]> for r < k, i < nv (h complex128 (k, nv));
-        w2[r] = ||w[r]||^2 (float64 device tensor) when given."""
+        w2[r] = ||w[r]||^2 (float64 device tensor) when given; ncol (int64
+        device tensor of k counts): r uses only V[r, :ncol[r]], h zero beyond."""
         n = V.shape[-1]
         part = self._part(k, nv, n, V.device)
         capi.check(capi.lib().gla_krylov_dots(self._code(V), k, nv, n, self._ptr(V), V.stride(0), V.stride(1),
                                               self._ptr(w), w.stride(0), self._ptr(part), part.numel() * 8,
-                                              self._ptr(h), self._ptr(w2), ctypes.c_void_p(stream)))
+                                              self._ptr(h), self._ptr(w2), self._ptr(ncol), ctypes.c_void_p(stream)))
 
-    def axpy(self, V, k, nv, h, w, s, stream, w2=None):
-        """w[r] += s * sum_i h[r, i] V[r, i] for r < k, i < nv; w2[r] =
-        ||w[r]||^2 afterwards when given."""
+    def axpy(self, V, k, nv, h, w, s, stream, w2=None, ncol=None):
+        """w[r] += s * sum_i h[r, i] V[r, i] for r < k, i < nv (i < ncol[r]
+        when given); w2[r] = ||w[r]||^2 afterwards when given."""
         n = V.shape[-1]
         part = self._part(k, 0, n, V.device) if w2 is not None else None
         capi.check(capi.lib().gla_krylov_axpy(self._code(V), k, nv, n, self._ptr(V), V.stride(0), V.stride(1),
                                               self._ptr(h), h.stride(0) if h is not None else 0, self._ptr(w),
                                               w.stride(0), float(s), self._ptr(part),
                                               0 if part is None else part.numel() * 8, self._ptr(w2),
+                                              self._ptr(ncol), ctypes.c_void_p(stream)))
+
+    def next(self, V, k, idx, w, w2, stream, stage=None):
+        """V[r, idx[r]] = w[r] / max(sqrt(w2[r]), 1e-30) for r < k (w2 the
+        float64 device squared norms), also into stage[r] when given."""
+        n = V.shape[-1]
+        capi.check(capi.lib().gla_krylov_next(self._code(V), k, n, self._ptr(V), V.stride(0), V.stride(1),
+                                              self._ptr(idx), self._ptr(w), w.stride(0), self._ptr(w2),
+                                              self._ptr(stage), 0 if stage is None else stage.stride(0),
                                               ctypes.c_void_p(stream)))
 
     def rows(self, V, k, idx, b, stream, scale=None):
@@ -163,25 +173,32 @@ def _stream():
 DGKS = 0.7071067811865476
 
 
-def gs_pass(kry, V, k, nv, w, stream):
-    """One classical Gram-Schmidt pass of w[:k] against V[:k, :nv].  Returns
-    (coefficients complex128 (k, nv), ||w||^2 before, ||w||^2 after) as
-    device tensors.  On the CPU (tests of the host logic) the same products
-    run in torch."""
+def gs_pass(kry, V, k, nv, w, stream, ncol=None, out=None):
+    """One classical Gram-Schmidt pass of w[:k] against V[:k, :nv] (against
+    V[r, :ncol[r]] for each r when ncol, an int64 device tensor, is given).
+    Returns (coefficients complex128 (k, nv), ||w||^2 before, ||w||^2 after)
+    as device tensors -- views of `out` (float64, at least k (2 nv + 2)
+    entries) when given, so one copy moves all three.  On the CPU (tests of
+    the host logic) the same products run in torch."""
     torch = _torch()
     if not V.is_cuda:
         Vj = V[:k, :nv].to(torch.complex128)
+        if ncol is not None:
+            keep = torch.arange(nv)[None, :] < ncol[:k, None]
+            Vj = Vj * keep[:, :, None]
         w64 = w[:k].to(torch.complex128)
         n0 = (w64.abs() ** 2).sum(1)
         h = torch.einsum("rin,rn->ri", Vj.conj(), w64)
         w[:k] -= torch.einsum("ri,rin->rn", h, Vj).to(w.dtype)
         n1 = (w[:k].to(torch.complex128).abs() ** 2).sum(1)
         return h, n0, n1
-    h = torch.empty((k, nv), dtype=torch.complex128, device=V.device)
-    n0 = torch.empty(k, dtype=torch.float64, device=V.device)
-    n1 = torch.empty_like(n0)
-    kry.dots(V, k, nv, w, h, stream, w2=n0)
-    kry.axpy(V, k, nv, h, w, -1.0, stream, w2=n1)
+    if out is None:
+        out = torch.empty(k * (2 * nv + 2), dtype=torch.float64, device=V.device)
+    h = out[:2 * k * nv].view(torch.complex128).view(k, nv)
+    n0 = out[2 * k * nv:2 * k * nv + k]
+    n1 = out[2 * k * nv + k:2 * k * nv + 2 * k]
+    kry.dots(V, k, nv, w, h, stream, w2=n0, ncol=ncol)
+    kry.axpy(V, k, nv, h, w, -1.0, stream, w2=n1, ncol=ncol)
     return h, n0, n1
 
 

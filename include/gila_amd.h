@@ -211,23 +211,34 @@ int64_t gla_krylov_dots_workspace(int64_t k, int64_t nv, int64_t n);
 
 /* h[r][i] = sum_q conj(V[r][i][q]) w[r][q], and w2[r] = ||w[r]||^2 when w2 is
  * not NULL (device array of k doubles); FP64 sums in a fixed order (the
- * result for one right-hand side does not depend on k). */
+ * result for one right-hand side does not depend on k).  ncol (device int64,
+ * k entries, or NULL for nv each): right-hand side r uses only its first
+ * ncol[r] vectors, and h[r][i] = 0 for i >= ncol[r]. */
 gla_status gla_krylov_dots(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* w, int64_t w_rhs_stride, void* part,
-                           int64_t part_bytes, void* h, double* w2, void* stream);
+                           int64_t part_bytes, void* h, double* w2, const int64_t* ncol, void* stream);
 
-/* w[r][q] += s * sum_i h[r][i] V[r][i][q]; then w2[r] = ||w[r]||^2 when w2 is
- * not NULL (part is then required, part may be NULL otherwise). */
+/* w[r][q] += s * sum_i h[r][i] V[r][i][q] over i < ncol[r] (i < nv when ncol
+ * is NULL); then w2[r] = ||w[r]||^2 when w2 is not NULL (part is then
+ * required, part may be NULL otherwise). */
 gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, const void* V, int64_t v_rhs_stride,
                            int64_t v_vec_stride, const void* h, int64_t h_rhs_stride, void* w,
                            int64_t w_rhs_stride, double s, void* part, int64_t part_bytes, double* w2,
-                           void* stream);
+                           const int64_t* ncol, void* stream);
 
 /* Row moves between a basis slab and a batch, idx[r] (device int64) the
  * vector of right-hand side r: scale == NULL gathers b[r] = V[r][idx[r]];
  * otherwise scatters V[r][idx[r]] = b[r] / scale[r] (device doubles). */
 gla_status gla_krylov_rows(int32_t dtype, int64_t k, int64_t n, void* V, int64_t v_rhs_stride, int64_t v_vec_stride,
                            const int64_t* idx, void* b, int64_t b_rhs_stride, const double* scale, void* stream);
+
+/* The next Arnoldi vector from the device-resident squared norm:
+ * V[r][idx[r]] = w[r] / max(sqrt(w2[r]), 1e-30), written also to stage[r]
+ * (the next matvec's input) when stage is not NULL.  Saves the host round
+ * trip of the norm and the gather of the next step. */
+gla_status gla_krylov_next(int32_t dtype, int64_t k, int64_t n, void* V, int64_t v_rhs_stride, int64_t v_vec_stride,
+                           const int64_t* idx, const void* w, int64_t w_rhs_stride, const double* w2, void* stage,
+                           int64_t stage_rhs_stride, void* stream);
 
 #ifdef __cplusplus
 }

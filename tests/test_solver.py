@@ -211,3 +211,68 @@ def test_krylov_kernels_match_host(gila, gpu, dt, tol):
     for r in range(k):
         got = Vs[r, int(idx[r])].cpu().to(torch.complex128)
         assert (got - w64[r] / float(sc[r])).abs().max() <= 1e-6 * w64[r].abs().max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dt", ["complex64", "complex128"])
+def test_krylov_column_counts_and_next_vector(gila, gpu, dt):
+    """Per right-hand-side column counts (a request in the middle of its own
+    restart cycle inside a batch): the products over V[r, :ncol[r]] equal
+    the full products with the columns beyond zeroed, bitwise, whatever the
+    stale columns hold; the next-vector kernel divides by the device norm
+    exactly as a host-side scale would and stages the same vector."""
+    import torch
+    print(f"case krylov column counts {dt}", flush=True)
+    tdt = getattr(torch, dt)
+    cap, m1, n, k, nv = 4, 12, 70001, 4, 11
+    g = torch.Generator().manual_seed(11)
+    V = (torch.randn((cap, m1, n), generator=g, dtype=torch.float64)
+         + 1j * torch.randn((cap, m1, n), generator=g, dtype=torch.float64)).to(tdt)
+    w = (torch.randn((cap, n), generator=g, dtype=torch.float64)
+         + 1j * torch.randn((cap, n), generator=g, dtype=torch.float64)).to(tdt)
+    cnt = [11, 3, 0, 9]                                   # full, partial, none, one sweep + 1
+    Vz = V.clone()
+    for r, c in enumerate(cnt):
+        Vz[r, c:] = 0
+    kry = gila.solve.KrylovOps()
+    ncol = torch.tensor(cnt, dtype=torch.int64, device="cuda")
+    Vd, Vzd, wd = V.cuda(), Vz.cuda(), w.cuda()
+    h = torch.full((k, nv), 7.0 + 7.0j, dtype=torch.complex128, device="cuda")
+    hz = torch.empty((k, nv), dtype=torch.complex128, device="cuda")
+    w2, w2z = (torch.empty(k, dtype=torch.float64, device="cuda") for _ in range(2))
+    kry.dots(Vd, k, nv, wd, h, 0, w2=w2, ncol=ncol)
+    kry.dots(Vzd, k, nv, wd, hz, 0, w2=w2z)
+    torch.cuda.synchronize()
+    assert torch.equal(h.cpu(), hz.cpu())
+    assert torch.equal(w2.cpu(), w2z.cpu())
+    for r, c in enumerate(cnt):
+        assert bool((h[r, c:] == 0).all())
+    upd, updz = wd.clone(), wd.clone()
+    n1, n1z = (torch.empty(k, dtype=torch.float64, device="cuda") for _ in range(2))
+    hh = (h * 0.25).contiguous()
+    kry.axpy(Vd, k, nv, hh, upd, -1.0, 0, w2=n1, ncol=ncol)
+    kry.axpy(Vzd, k, nv, hh, updz, -1.0, 0, w2=n1z)
+    torch.cuda.synchronize()
+    assert torch.equal(upd.cpu(), updz.cpu())
+    assert torch.equal(n1.cpu(), n1z.cpu())
+    assert torch.equal(upd[2].cpu(), w[2])                # no columns: unchanged
+    # next vector: V[r, idx[r]] = upd[r] / sqrt(n1[r]) and the staged copy;
+    # the same as the scatter with a host-computed scale
+    idx = torch.tensor([5, 0, 11, 2], dtype=torch.int64, device="cuda")
+    Va, Vb = Vd.clone(), Vd.clone()
+    stage = torch.zeros((cap, n), dtype=tdt, device="cuda")
+    kry.next(Va, k, idx, upd, n1, 0, stage=stage)
+    sc = torch.from_numpy(np.maximum(np.sqrt(n1.cpu().numpy()), 1e-30)).cuda()
+    kry.rows(Vb, k, idx, upd, 0, scale=sc)
+    torch.cuda.synchronize()
+    assert torch.equal(Va.cpu(), Vb.cpu())
+    for r in range(k):
+        assert torch.equal(stage[r].cpu(), Va[r, int(idx[r])].cpu())
+    assert bool((stage[k:] == 0).all())
+    # a zero vector normalizes by the floor, not by zero
+    z = torch.zeros((1, n), dtype=tdt, device="cuda")
+    zn = torch.zeros(1, dtype=torch.float64, device="cuda")
+    Vc = Vd.clone()
+    kry.next(Vc, 1, idx[:1], z, zn, 0)
+    torch.cuda.synchronize()
+    assert bool((Vc[0, 5] == 0).all())
