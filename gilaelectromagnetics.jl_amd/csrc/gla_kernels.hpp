@@ -47,11 +47,57 @@ __device__ __forceinline__ void axis_lookup(const AxisMap& a, int p, int k, int&
 // gb points at the column's entry of the branch in the device layout
 // gf[((b U3 + j3) U2 U1 + j2 U1 + j1) * 6 + comp], uu = U1 U2, so each point
 // adds j3 * uu records; s1, s2 are the column's mirror signs.
-template <typename R, int E, int TP>
+// The six stored components of point e of a column (and its axis-3 mirror
+// sign), for the pipelined multiply below.
+template <typename R, int TP>
+__device__ __forceinline__ void g_fetch(const cpx<R>* __restrict__ gb, int uu, const AxisMap& a3, int p3, int t, int e,
+                                        cpx<R> (&d)[6], R& s3f)
+{
+    int j3, s3;
+    axis_lookup(a3, p3, t + TP * e, j3, s3);
+    s3f = R(s3);
+    const cpx<R>* g = gb + (int64_t)(j3 * uu) * 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = g[k];
+}
+
+// PIPE: the six loads of point e + 1 are issued before point e is
+// multiplied, so one memory latency is exposed per call instead of one per
+// point; with `first` the caller has already fetched point 0 (g_fetch), for
+// instance before the forward transform, whose stages then hide its latency.
+template <typename R, int E, int TP, bool PIPE = false>
 __device__ __forceinline__ void g_multiply(cpx<R> (&w)[3][E], const cpx<R>* __restrict__ gb, int uu,
-                                           const AxisMap& a3, int p3, int t, int s1, int s2, bool act)
+                                           const AxisMap& a3, int p3, int t, int s1, int s2, bool act,
+                                           const cpx<R> (*first)[6] = nullptr, R first_s3 = R(1))
 {
     const R s12 = R(s1 * s2);
+    if constexpr (PIPE) {
+        cpx<R> gv[2][6];
+        R s3v[2];
+        auto fetch = [&](int e, cpx<R>(&d)[6], R& s3f) { g_fetch<R, TP>(gb, uu, a3, p3, t, e, d, s3f); };
+        if (!act) return;
+        if (first) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gv[0][k] = (*first)[k];
+            s3v[0] = first_s3;
+        } else {
+            fetch(0, gv[0], s3v[0]);
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (e + 1 < E) fetch(e + 1, gv[(e + 1) & 1], s3v[(e + 1) & 1]);
+            __builtin_amdgcn_sched_barrier(0);
+            const cpx<R>* g = gv[e & 1];
+            const R s13 = R(s1) * s3v[e & 1], s23 = R(s2) * s3v[e & 1];
+            const cpx<R> gxy = s12 * g[3], gxz = s13 * g[4], gyz = s23 * g[5];
+            const cpx<R> x0 = w[0][e], x1 = w[1][e], x2 = w[2][e];
+            w[0][e] = g[0] * x0 + gxy * x1 + gxz * x2;
+            w[1][e] = gxy * x0 + g[1] * x1 + gyz * x2;
+            w[2][e] = gxz * x0 + gyz * x1 + g[2] * x2;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int k3 = t + TP * e;
@@ -427,24 +473,50 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
 
     const int uu = a1.U * a2.U;
     const int gcol = j2 * a1.U + j1;
-    auto mul_g = [&](cpx<R>(&w)[3][E], int p3) {
+    // GG bit 1: G^ loads pipelined one point ahead; bit 2 in addition: the
+    // first point's loads issued before the forward transform (pre_g), so
+    // they arrive during its stages
+    constexpr bool GPIPE = (GG & 2) != 0, GPRE = GPIPE && (GG & 4) != 0;
+    cpx<R> g0[6];
+    R g0s = R(1);
+    auto gbase = [&](int p3) {
         const int b = (p1 << 2) | (p2 << 1) | p3;
-        g_multiply<R, E, TP>(w, gf + ((int64_t)b * a3.U * uu + gcol) * 6, uu, a3, p3, t, s1, s2, act);
+        return gf + ((int64_t)b * a3.U * uu + gcol) * 6;
     };
+    auto pre_g = [&](int p3) {
+        if constexpr (GPRE)
+            if (act) g_fetch<R, TP>(gbase(p3), uu, a3, p3, t, 0, g0, g0s);
+    };
+    auto mul_g = [&](cpx<R>(&w)[3][E], int p3) {
+        g_multiply<R, E, TP, GPIPE>(w, gbase(p3), uu, a3, p3, t, s1, s2, act, GPRE ? &g0 : nullptr, g0s);
+    };
+    // GG bit 0: the transform's root table and the split phases staged in
+    // LDS once per workgroup (their reads inside the stages and the split /
+    // merge then wait on LDS, not on the L2)
+    __shared__ cpx<R> twl[(GG & 1) ? 2 * N : 1];
+    if constexpr ((GG & 1) != 0) {
+        for (int i = threadIdx.x; i < N; i += TP * PX) {
+            twl[i] = tw[i];
+            twl[N + i] = phz[i];
+        }
+        __syncthreads();
+    }
+    const cpx<R>* twu = (GG & 1) ? twl : tw;
+    const cpx<R>* phu = (GG & 1) ? twl + N : phz;
     auto fwd = [&](cpx<R>(&w)[3][E]) {
         if constexpr (KB == 3) {
-            FftF3::run(w, lds, t, xl, tw);
+            FftF3::run(w, lds, t, xl, twu);
         } else {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) FftF1::run(w[c], lds, t, xl, tw);
+            for (int c = 0; c < 3; ++c) FftF1::run(w[c], lds, t, xl, twu);
         }
     };
     auto inv = [&](cpx<R>(&w)[3][E]) {
         if constexpr (KB == 3) {
-            FftI3::run(w, lds, t, xl, tw);
+            FftI3::run(w, lds, t, xl, twu);
         } else {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) FftI1::run(w[c], lds, t, xl, tw);
+            for (int c = 0; c < 3; ++c) FftI1::run(w[c], lds, t, xl, twu);
         }
     };
 
@@ -462,11 +534,13 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                 if constexpr (MODE == 1)
                     b = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
                 wv[0][c][e] = a + b;
-                wv[1][c][e] = phz[n] * (a - b);
+                wv[1][c][e] = phu[n] * (a - b);
             }
+        pre_g(0);
         fwd(wv[0]);
         mul_g(wv[0], 0);
         inv(wv[0]);
+        pre_g(1);
         fwd(wv[1]);
         mul_g(wv[1], 1);
         inv(wv[1]);
@@ -477,7 +551,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int n = t + TP * e;
-                const cpx<R> po = cmulc(wv[1][c][e], phz[n]);
+                const cpx<R> po = cmulc(wv[1][c][e], phu[n]);
                 if (n < top) col[c * cs + (int64_t)n * zs] = wv[0][c][e] + po;
                 if constexpr (MODE == 2)
                     if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = wv[0][c][e] - po;
@@ -505,8 +579,9 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                     a = xa[c][e];
                     if constexpr (MODE == 1) b = xb[c][e];
                 }
-                w[c][e] = p3 == 0 ? a + b : phz[n] * (a - b);
+                w[c][e] = p3 == 0 ? a + b : phu[n] * (a - b);
             }
+        pre_g(p3);
         fwd(w);
         mul_g(w, p3);
         inv(w);
@@ -519,7 +594,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                     if constexpr (MODE == 2) accb[c][e] = w[c][e];
                 } else {
                     const int n = t + TP * e;
-                    const cpx<R> po = cmulc(w[c][e], phz[n]);
+                    const cpx<R> po = cmulc(w[c][e], phu[n]);
                     acc[c][e] = acc[c][e] + po;
                     if constexpr (MODE == 2) accb[c][e] = accb[c][e] - po;
                 }

@@ -34,8 +34,23 @@
 #ifndef GLA_ZMINW
 #define GLA_ZMINW 1
 #endif
-#ifndef GLA_ZGG
-#define GLA_ZGG 64
+// resident waves per SIMD asked of the compiler for pass 3, per element type
+#ifndef GLA_ZMINW_F
+#define GLA_ZMINW_F GLA_ZMINW
+#endif
+#ifndef GLA_ZMINW_D
+#define GLA_ZMINW_D GLA_ZMINW
+#endif
+// pass 3 options (k_zpass GG): bit 0 stages the root table and the split
+// phases in LDS, bit 1 pipelines the G^ loads one point ahead, bit 2 issues
+// the first point's G^ loads before the forward transform
+// (measured at 128^3: FP32 215 -> 189-196 us with 7, FP64 403 -> 355-358 us
+// with 3; bit 2 costs FP64 registers)
+#ifndef GLA_ZGG_F
+#define GLA_ZGG_F 7
+#endif
+#ifndef GLA_ZGG_D
+#define GLA_ZGG_D 3
 #endif
 #ifndef GLA_ZPADSH
 #define GLA_ZPADSH 3
@@ -90,6 +105,8 @@ struct Cfg {
     static constexpr int PX = DBL ? GLA_PXY_D : GLA_PXY_F;   // axis-2 passes: columns per bundle
     static constexpr int E_ROW = GLA_EROW;            // passes 1, 2, 4, 5
     static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
+    static constexpr int ZMINW = DBL ? GLA_ZMINW_D : GLA_ZMINW_F;
+    static constexpr int ZGG = DBL ? GLA_ZGG_D : GLA_ZGG_F;
 };
 
 
@@ -199,9 +216,9 @@ static hipError_t launch_zpass_chunk(const Geom& g, const DevTab& tb, void* W2, 
         dim3 pgrid(8 * nb_xcd, 1, nrhs);
         if (mode == 0) hipLaunchKernelGGL((k_zpipe<R, N, K::TP_Z, PX, 0, K::KB_Z, PF>), pgrid, blk, 0, st, GLA_ZARGS);
         else hipLaunchKernelGGL((k_zpipe<R, N, K::TP_Z, PX, 2, K::KB_Z, PF>), pgrid, blk, 0, st, GLA_ZARGS);
-    } else if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
-    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
-    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, GLA_ZMINW, K::KB_Z, SC, K::WL_Z, GLA_ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    } else if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
 #undef GLA_ZARGS
     return hipGetLastError();
 }
