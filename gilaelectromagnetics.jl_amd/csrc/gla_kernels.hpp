@@ -21,7 +21,27 @@
 #include <type_traits>
 #include "gla_fft.hpp"
 
+// passes 1, 2, 4, 5: the root table and split phases of the pass's length
+// read from an LDS copy made at the start of each workgroup (1) or from
+// global memory (0)
+#ifndef GLA_XYTAB
+#define GLA_XYTAB 1
+#endif
+
 namespace gla {
+
+// Copies the length-N root table (tab[0, N)) and split phases (tab[N, 2N))
+// of an axis into LDS for the workgroup's nt threads.
+template <typename R, int N>
+__device__ __forceinline__ void stage_tables(cpx<R>* tab, const cpx<R>* __restrict__ tw,
+                                             const cpx<R>* __restrict__ phz, int nt)
+{
+    for (int i = threadIdx.x; i < N; i += nt) {
+        tab[i] = tw[i];
+        tab[N + i] = phz[i];
+    }
+    __syncthreads();
+}
 
 // per-axis description of how a branch frequency k maps onto stored G^
 // entries (restating mulBrnHst!, src/glaActSup.jl:691-753): on a symmetric
@@ -133,6 +153,12 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
     // rows are whole pencils inside one wave when TP divides 64
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
     __shared__ cpx<R> lds[2 * PEN * PS];
+    __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
+    if constexpr (GLA_XYTAB != 0) {
+        stage_tables<R, N>(tab, tw, phz, TP * PEN);
+        tw = tab;
+        phz = tab + N;
+    }
     const int t = threadIdx.x % TP, pen = threadIdx.x / TP;
     const int rows = S2 * S3 * 3;
     const int row = blockIdx.x * PEN + pen;
@@ -272,6 +298,12 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     constexpr int PS = N + (N >> PADSH) + 1;
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
     __shared__ cpx<R> lds[2 * PEN * PS];
+    __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
+    if constexpr (GLA_XYTAB != 0) {
+        stage_tables<R, N>(tab, tw, phz, TP * PEN);
+        tw = tab;
+        phz = tab + N;
+    }
     const int t = threadIdx.x % TP, pen = threadIdx.x / TP;
     const int rows = T2 * T3 * 3;
     const int row = blockIdx.x * PEN + pen;
@@ -318,6 +350,12 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restric
 {
     constexpr int E = N / TP;
     __shared__ cpx<R> lds[2 * PX * N];
+    __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
+    if constexpr (GLA_XYTAB != 0) {
+        stage_tables<R, N>(tab, tw, phz, TP * PX);
+        tw = tab;
+        phz = tab + N;
+    }
     const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
     const int m1 = blockIdx.x * PX + xl;
     const bool act = m1 < L1;
@@ -355,6 +393,12 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
 {
     constexpr int E = N / TP;
     __shared__ cpx<R> lds[2 * PX * N];
+    __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
+    if constexpr (GLA_XYTAB != 0) {
+        stage_tables<R, N>(tab, tw, phz, TP * PX);
+        tw = tab;
+        phz = tab + N;
+    }
     const int xl = threadIdx.x % PX, t = threadIdx.x / PX;
     const int m1 = blockIdx.x * PX + xl;
     const bool act = m1 < L1;
