@@ -43,9 +43,10 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched right-hand-side leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served solve-stream leg")
-    ap.add_argument("--serve-requests", type=int, default=30)
-    ap.add_argument("--serve-rate", type=float, default=4.0,
-                    help="Poisson arrival rate of the served stream, in multiples of the sequential FP64 rate")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rates", default="2,4,8",
+                    help="Poisson arrival rates of the served stream (comma-separated), in multiples of the "
+                         "measured sequential FP64 rate; the last is the headline")
     ap.add_argument("--counters", choices=["c64", "c128"], default=None,
                     help="short program for rocprofv3 counter passes: a 512 MiB device copy (the calibration "
                          "of FETCH_SIZE/WRITE_SIZE) and three matvecs of the given type; prints no JSON")
@@ -238,11 +239,14 @@ def batched_leg(gila, log):
 def serve_leg(gila, a, log):
     """Served solve stream against sequential plain FP64 solves of the same
     requests (GMRES(30) on (I - chi G) E = E_inc to 1e-6 by the FP64
-    residual)."""
+    residual): the sequential rate first, then the same requests through the
+    server with Poisson arrivals at each rate of the sweep (one warm server,
+    operators resident across the sweep)."""
     S = gila.serve
     shapes = [(32, 32, 32), (64, 64, 48), (128, 64, 64), (128, 128, 128), (256, 128, 64)]
     from gila_amd import inputs
-    u = inputs.uniform(31337, 2 * a.serve_requests)
+    rates = [float(v) for v in a.serve_rates.split(",") if v.strip()]
+    u = inputs.uniform(31337, a.serve_requests * (1 + len(rates)))
 
     def requests():
         return [S.random_request(20000 + i, shapes[int(u[i] * len(shapes))], tol=1e-6)
@@ -259,30 +263,42 @@ def serve_leg(gila, a, log):
     seq.run(reqs, on_done=lambda r: log(f"  seq {r.rid} {'x'.join(map(str, r.cells))} steps={r.steps}"))
     t_seq = time.perf_counter() - t0
     rate_seq = len(reqs) / t_seq
-    lam = a.serve_rate * rate_seq
-    gaps = -np.log(u[a.serve_requests:]) / lam
-    arrivals = np.cumsum(gaps) - gaps[0]
-    reqs2 = requests()
-    log(f"serve: served stream, Poisson arrivals at {lam:.2f}/s ({a.serve_rate}x the sequential rate)")
-    srv.run(reqs2, arrivals=list(arrivals),
-            on_done=lambda r: log(f"  srv {r.rid} {'x'.join(map(str, r.cells))} steps={r.steps} "
-                                  f"lat={r.latency:.2f}s"))
-    st = S.latency_stats(reqs2)
     st_seq = S.latency_stats(reqs)
-    agree = max(float(np.linalg.norm(r2.result - r1.result) / np.linalg.norm(r1.result))
-                for r1, r2 in zip(reqs, reqs2))
+    sweep = []
+    for k, mult in enumerate(rates):
+        lam = mult * rate_seq
+        uu = u[a.serve_requests * (1 + k):a.serve_requests * (2 + k)]
+        gaps = -np.log(uu) / lam
+        arrivals = np.cumsum(gaps) - gaps[0]
+        reqs2 = requests()
+        log(f"serve: served stream, Poisson arrivals at {lam:.2f}/s ({mult}x the sequential rate)")
+        srv.run(reqs2, arrivals=list(arrivals),
+                on_done=lambda r: log(f"  srv {r.rid} {'x'.join(map(str, r.cells))} steps={r.steps} "
+                                      f"lat={r.latency:.2f}s"))
+        st = S.latency_stats(reqs2)
+        agree = max(float(np.linalg.norm(r2.result - r1.result) / np.linalg.norm(r1.result))
+                    for r1, r2 in zip(reqs, reqs2))
+        sweep.append({"rate_x_sequential": mult, "arrival_rate_per_s": lam, "solves_per_s": st["solves_per_s"],
+                      "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
+                      "speedup_vs_sequential_fp64": st["solves_per_s"] / rate_seq,
+                      "max_rel_diff_served_vs_sequential": agree,
+                      "all_converged": bool(all(r.converged for r in reqs2)),
+                      "max_relres": max(r.relres for r in reqs2), "steps_total": sum(r.steps for r in reqs2)})
+    head = sweep[-1]
     return {"workload": f"{len(reqs)} requests, domains drawn (seeded) from "
                         + ", ".join("x".join(map(str, s)) for s in shapes)
                         + " cells at 1/32 wavelength; chi 0 outside a random box of 10-40% of the domain, "
-                          "U[0.5,3]+i U[0,0.1] inside; E_inc a plane wave at a random angle",
-            "solves_per_s": st["solves_per_s"], "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
-            "arrival_rate_per_s": lam,
+                          "U[0.5,3]+i U[0,0.1] inside; E_inc a plane wave at a random angle; Poisson arrivals "
+                          "at each rate of the sweep (headline: the last)",
+            "solves_per_s": head["solves_per_s"], "p50_latency_s": head["p50_latency_s"],
+            "p99_latency_s": head["p99_latency_s"], "arrival_rate_per_s": head["arrival_rate_per_s"],
             "sequential_fp64_solves_per_s": rate_seq, "sequential_fp64_p50_latency_s": st_seq["p50_s"],
-            "speedup_vs_sequential_fp64": st["solves_per_s"] / rate_seq,
-            "max_rel_diff_served_vs_sequential": agree,
-            "all_converged": bool(all(r.converged for r in reqs + reqs2)),
-            "max_relres": max(r.relres for r in reqs + reqs2),
-            "steps_total": sum(r.steps for r in reqs2)}
+            "sequential_fp64_all_converged": bool(all(r.converged for r in reqs)),
+            "speedup_vs_sequential_fp64": head["speedup_vs_sequential_fp64"],
+            "max_rel_diff_served_vs_sequential": max(r["max_rel_diff_served_vs_sequential"] for r in sweep),
+            "all_converged": bool(all(r["all_converged"] for r in sweep)),
+            "max_relres": max(r["max_relres"] for r in sweep),
+            "sweep": sweep}
 
 
 def main():

@@ -25,11 +25,9 @@ constexpr ZShape ztune(int complex_bytes, int n)
             case 16: return {4, 16};
             case 20: return {4, 16};
             case 24: return {4, 16};
-            case 32: return {4, 8};
             case 40: return {4, 16};
             case 48: return {4, 16};
             case 80: return {4, 8};
-            case 96: return {4, 8};
             case 192: return {8, 8};
             case 256: return {8, 8};
             case 320: return {4, 8};

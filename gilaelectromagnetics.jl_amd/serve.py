@@ -673,11 +673,12 @@ class Server:
         completion order."""
         self.on_done = on_done
         t0 = time.perf_counter()
+        n0 = len(self.done)        # a server may run several streams in turn
         order = list(range(len(requests)))
         if arrivals is not None:
             order.sort(key=lambda i: arrivals[i])
         nxt = 0
-        while len(self.done) < len(requests):
+        while len(self.done) - n0 < len(requests):
             now = time.perf_counter() - t0
             while nxt < len(order) and (arrivals is None or arrivals[order[nxt]] <= now):
                 i = order[nxt]
@@ -691,7 +692,7 @@ class Server:
                 raise RuntimeError("a request does not fit the memory budget")
             if timeout is not None and time.perf_counter() - t0 > timeout:
                 raise TimeoutError("serving did not finish in time")
-        return self.done
+        return self.done[n0:]
 
 
 def latency_stats(reqs):
