@@ -29,7 +29,7 @@ EXPORTED = (
     "gla_opr_export_fur", "gla_opr_get_dims", "gla_opr_adjoint", "gla_opr_destroy", "gla_opr_apply",
     "gla_opr_apply_axpby", "gla_opr_apply_host", "gla_opr_workspace_bytes", "gla_opr_profile",
     "gla_ls_apply", "gla_krylov_dots_workspace", "gla_krylov_dots", "gla_krylov_axpy",
-    "gla_krylov_rows", "gla_krylov_next", "gla_opr_get_vols", "gla_opr_get_opts", "gla_opr_save", "gla_opr_load",
+    "gla_krylov_rows", "gla_krylov_next", "gla_krylov_givens", "gla_opr_get_vols", "gla_opr_get_opts", "gla_opr_save", "gla_opr_load",
 )
 
 
@@ -89,6 +89,7 @@ def load(path=LIB_PATH):
         "gla_krylov_axpy": [i32, i64, i64, i64, vp, i64, i64, vp, i64, vp, i64, d, vp, i64, vp, vp, vp],
         "gla_krylov_rows": [i32, i64, i64, vp, i64, i64, vp, vp, i64, vp, vp],
         "gla_krylov_next": [i32, i64, i64, vp, i64, i64, vp, vp, i64, vp, vp, i64, vp],
+        "gla_krylov_givens": [i64, i64, vp, vp, i64, vp, vp, vp, vp, vp],
         "gla_opr_get_vols": [vp, P(gla_vol), P(gla_vol)],
         "gla_opr_get_opts": [vp, P(gla_opts)],
         "gla_opr_save": [vp, ctypes.c_char_p],

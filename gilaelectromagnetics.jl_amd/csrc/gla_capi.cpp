@@ -813,6 +813,66 @@ gla_status gla_krylov_rows(int32_t dtype, int64_t k, int64_t n, void* V, int64_t
     return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, hipGetErrorString(e));
 }
 
+gla_status gla_krylov_givens(int64_t k, int64_t m, const int64_t* js, const double* hcol, int64_t ldh,
+                             const double* hn, double* cs, double* sn, double* H, double* g)
+{
+    if (k < 0 || m < 1) return fail(GLA_ERR_ARG, "k >= 0 and m >= 1 required");
+    if (k == 0) return GLA_OK;
+    if (!js || !hcol || !hn || !cs || !sn || !H || !g) return fail(GLA_ERR_ARG, "null argument");
+    for (int64_t s = 0; s < k; ++s)
+        if (js[s] < 0 || js[s] >= m || js[s] >= ldh)
+            return fail(GLA_ERR_ARG, "step index outside [0, min(m, ldh))");
+    struct Z {
+        double r, i;
+    };
+    auto mul = [](Z a, Z b) { return Z{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; };
+    auto add = [](Z a, Z b) { return Z{a.r + b.r, a.i + b.i}; };
+    auto scl = [](double c, Z a) { return Z{c * a.r, c * a.i}; };
+    auto cnj = [](Z a) { return Z{a.r, -a.i}; };
+    std::vector<Z> col((size_t)m + 1);
+    for (int64_t s = 0; s < k; ++s) {
+        const int64_t j = js[s];
+        const Z* hc = (const Z*)hcol + s * ldh;
+        Z* snz = (Z*)sn + s * m;
+        double* csr = cs + s * m;
+        for (int64_t i = 0; i <= j; ++i) col[(size_t)i] = hc[i];
+        col[(size_t)j + 1] = Z{hn[s], 0.0};
+        // earlier rotations
+        for (int64_t i = 0; i < j; ++i) {
+            const double c = csr[i];
+            const Z z = snz[i], a = col[(size_t)i], b = col[(size_t)i + 1];
+            col[(size_t)i] = add(scl(c, a), mul(z, b));
+            col[(size_t)i + 1] = add(scl(-1.0, mul(cnj(z), a)), scl(c, b));
+        }
+        // the rotation that zeroes the subdiagonal entry
+        const Z a = col[(size_t)j], b = col[(size_t)j + 1];
+        const double aa = std::hypot(a.r, a.i), den = std::hypot(aa, std::hypot(b.r, b.i));
+        double c;
+        Z z;
+        if (den == 0.0) {
+            c = 1.0;
+            z = Z{0.0, 0.0};
+        } else if (aa == 0.0) {
+            c = 0.0;
+            z = Z{1.0, 0.0};
+        } else {
+            c = aa / den;
+            z = scl(1.0 / den, mul(scl(1.0 / aa, a), cnj(b)));
+        }
+        csr[j] = c;
+        snz[j] = z;
+        col[(size_t)j] = add(scl(c, a), mul(z, b));
+        col[(size_t)j + 1] = Z{0.0, 0.0};
+        Z* Hs = (Z*)H + s * (m + 1) * m;
+        for (int64_t i = 0; i <= j + 1; ++i) Hs[i * m + j] = col[(size_t)i];
+        Z* gs = (Z*)g + s * (m + 1);
+        const Z gj = gs[j];
+        gs[j + 1] = scl(-1.0, mul(cnj(z), gj));
+        gs[j] = scl(c, gj);
+    }
+    return GLA_OK;
+}
+
 gla_status gla_krylov_next(int32_t dtype, int64_t k, int64_t n, void* V, int64_t v_rhs_stride, int64_t v_vec_stride,
                            const int64_t* idx, const void* w, int64_t w_rhs_stride, const double* w2, void* stage,
                            int64_t stage_rhs_stride, void* stream)

@@ -348,14 +348,16 @@ class _Grid:
         return self._advance(hcol, hn, k, jm)
 
     def _advance(self, hcol, hn, k, jm):
-        """Givens update with the step's coefficients, then the slots whose
-        cycle ends."""
+        """Givens update with the step's coefficients (the library's host
+        routine gla_krylov_givens, one code path whatever the batch), then
+        the slots whose cycle ends."""
         m = self.m
-        js = self.j[:k].copy()
-        if k <= 4:
-            self._rotate_scalar(hcol, hn, js, jm)
-        else:
-            self._rotate_vector(hcol, hn, js, jm)
+        js = np.ascontiguousarray(self.j[:k], dtype=np.int64)
+        hcol = np.ascontiguousarray(hcol, dtype=np.complex128)
+        hn = np.ascontiguousarray(hn, dtype=np.float64)
+        ptr = lambda a: ctypes.c_void_p(a.ctypes.data)
+        capi.check(capi.lib().gla_krylov_givens(k, m, ptr(js), ptr(hcol), hcol.shape[1], ptr(hn), ptr(self.cs),
+                                                ptr(self.sn), ptr(self.H), ptr(self.g)))
         ar_h = np.arange(k)
         res = np.abs(self.g[ar_h, js + 1])
         est = res * self.rn[:k] / self.bn[:k]
@@ -367,63 +369,6 @@ class _Grid:
             if js[s] + 1 == m or res[s] <= self.inner_tol or est[s] <= 0.5 * req.tol:
                 ended.append(s)
         return ended
-
-    def _rotate_scalar(self, hcol, hn, js, jm):
-        """Givens update of each slot's Hessenberg column with Python complex
-        scalars (small batches)."""
-        for s in range(len(js)):
-            j = int(js[s])
-            col = [complex(v) for v in hcol[s, :j + 1]] + [complex(hn[s])]
-            cs, sn = self.cs[s], self.sn[s]
-            for i in range(j):
-                c, z = float(cs[i]), complex(sn[i])
-                a, b = col[i], col[i + 1]
-                col[i] = c * a + z * b
-                col[i + 1] = -z.conjugate() * a + c * b
-            a, b = col[j], col[j + 1]
-            den = (abs(a) ** 2 + abs(b) ** 2) ** 0.5
-            if den == 0:
-                c, z = 1.0, 0j
-            elif abs(a) == 0:
-                c, z = 0.0, 1 + 0j
-            else:
-                c, z = abs(a) / den, (a / abs(a)) * b.conjugate() / den
-            cs[j], sn[j] = c, z
-            col[j] = c * a + z * b
-            col[j + 1] = 0j
-            self.H[s, :j + 2, j] = col
-            gj = complex(self.g[s, j])
-            self.g[s, j + 1] = -z.conjugate() * gj
-            self.g[s, j] = c * gj
-
-    def _rotate_vector(self, hcol, hn, js, jm):
-        """The same update for all slots at once (numpy over the batch)."""
-        k = len(js)
-        ar_h = np.arange(k)
-        cols = np.zeros((k, self.m + 1), np.complex128)
-        cols[:, :jm + 1] = np.where(np.arange(jm + 1)[None, :] <= js[:, None], hcol[:, :jm + 1], 0)
-        cols[ar_h, js + 1] = hn
-        for i in range(jm):
-            act = i < js
-            c, sn = self.cs[:k, i], self.sn[:k, i]
-            a, b = cols[:, i].copy(), cols[:, i + 1].copy()
-            cols[:, i] = np.where(act, c * a + sn * b, a)
-            cols[:, i + 1] = np.where(act, -np.conj(sn) * a + c * b, b)
-        a, b = cols[ar_h, js], cols[ar_h, js + 1]
-        den = np.hypot(np.abs(a), np.abs(b))
-        aa = np.abs(a)
-        safe = np.where(aa > 0, aa, 1.0)
-        dd = np.where(den > 0, den, 1.0)
-        c = np.where(den == 0, 1.0, np.where(aa == 0, 0.0, aa / dd))
-        sn = np.where(den == 0, 0.0, np.where(aa == 0, 1.0, (a / safe) * np.conj(b) / dd))
-        self.cs[ar_h, js] = c
-        self.sn[ar_h, js] = sn
-        cols[ar_h, js] = c * a + sn * b
-        cols[ar_h, js + 1] = 0.0
-        self.H[ar_h, :, js] = cols
-        gj = self.g[ar_h, js].copy()
-        self.g[ar_h, js + 1] = -np.conj(sn) * gj
-        self.g[ar_h, js] = c * gj
 
     def launch_refine(self, slots):
         """FP64 update x += V y |r| and residual r = b - A x for the slots whose

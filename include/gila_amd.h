@@ -232,6 +232,20 @@ gla_status gla_krylov_axpy(int32_t dtype, int64_t k, int64_t nv, int64_t n, cons
 gla_status gla_krylov_rows(int32_t dtype, int64_t k, int64_t n, void* V, int64_t v_rhs_stride, int64_t v_vec_stride,
                            const int64_t* idx, void* b, int64_t b_rhs_stride, const double* scale, void* stream);
 
+/* Host routine (no device work): the Givens update of k restarted-GMRES
+ * slots after one Arnoldi step.  Complex values are interleaved (re, im)
+ * doubles in row-major host arrays: hcol (k, ldh) the new Hessenberg columns
+ * (entries 0..js[s]), hn (k) the subdiagonal norms, cs (k, m) real and
+ * sn (k, m) complex the slots' rotations, H (k, m+1, m) the rotated
+ * Hessenberg matrices and g (k, m+1) the rotated residual vectors.  Slot s
+ * at step j = js[s] applies rotations 0..j-1 to its column, forms rotation j
+ * (zeroing the subdiagonal), stores the column into H[s][0..j+1][j] and
+ * rotates g[s][j], g[s][j+1].  The reference has no solver; this is the
+ * least-squares update of the GMRES that its usage guide calls
+ * (docs/src/usage.md:370-391). */
+gla_status gla_krylov_givens(int64_t k, int64_t m, const int64_t* js, const double* hcol, int64_t ldh,
+                             const double* hn, double* cs, double* sn, double* H, double* g);
+
 /* The next Arnoldi vector from the device-resident squared norm:
  * V[r][idx[r]] = w[r] / max(sqrt(w2[r]), 1e-30), written also to stage[r]
  * (the next matvec's input) when stage is not NULL.  Saves the host round

@@ -276,3 +276,79 @@ def test_krylov_column_counts_and_next_vector(gila, gpu, dt):
     kry.next(Vc, 1, idx[:1], z, zn, 0)
     torch.cuda.synchronize()
     assert bool((Vc[0, 5] == 0).all())
+
+
+def _givens(gila, k, m, js, hcol, hn, cs, sn, H, g):
+    import ctypes
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    gila._capi.check(gila.lib().gla_krylov_givens(k, m, p(js), p(hcol), hcol.shape[1], p(hn), p(cs), p(sn),
+                                                  p(H), p(g)))
+
+
+def test_krylov_givens_solves_the_hessenberg_least_squares(gila):
+    """The host Givens update (gla_krylov_givens), step by step on random
+    Hessenberg columns of several slots at different steps: after step j,
+    |g[j+1]| equals the least-squares residual min ||beta e1 - Hbar y||
+    (numpy lstsq on the unrotated Hessenberg), back substitution with the
+    rotated R gives the lstsq solution, the rotated column has a zero
+    subdiagonal, and a slot's numbers do not depend on the other slots."""
+    rng = np.random.default_rng(5)
+    m, k = 12, 3
+    starts = [0, 4, 9]                                  # slots join at different steps
+    cs = np.zeros((k, m))
+    sn = np.zeros((k, m), np.complex128)
+    H = np.zeros((k, m + 1, m), np.complex128)
+    g = np.zeros((k, m + 1), np.complex128)
+    beta = np.array([2.0, 0.5, 3.0])
+    g[:, 0] = beta
+    Hbar = np.zeros((k, m + 1, m), np.complex128)     # unrotated
+    js = np.zeros(k, np.int64)
+    solo = None
+    for step in range(m + max(starts)):
+        live = [s for s in range(k) if starts[s] <= step and js[s] < m]
+        if not live:
+            continue
+        cols = np.zeros((k, m), np.complex128)
+        hn = np.zeros(k)
+        for s in live:
+            j = js[s]
+            cols[s, :j + 1] = rng.standard_normal(j + 1) + 1j * rng.standard_normal(j + 1)
+            hn[s] = abs(rng.standard_normal()) + 0.1
+            Hbar[s, :j + 1, j] = cols[s, :j + 1]
+            Hbar[s, j + 1, j] = hn[s]
+        # the live slots as one batch (slot order kept), then slot 1 alone
+        idx = np.array(live)
+        sub = [np.ascontiguousarray(a[idx]) for a in (cs, sn, H, g)]
+        if 1 in live:
+            one = [np.ascontiguousarray(a[1:2]) for a in (cs, sn, H, g)]
+            _givens(gila, 1, m, js[1:2].copy(), np.ascontiguousarray(cols[1:2]), hn[1:2].copy(), *one)
+        _givens(gila, len(live), m, js[idx].copy(), np.ascontiguousarray(cols[idx]), hn[idx].copy(), *sub)
+        for a, b in zip((cs, sn, H, g), sub):
+            a[idx] = b
+        if 1 in live:
+            pos = live.index(1)
+            for a, b in zip(sub, one):
+                assert np.array_equal(a[pos], b[0])
+        for s in live:
+            j = js[s]
+            assert H[s, j + 1, j] == 0
+            e1 = np.zeros(j + 2, np.complex128)
+            e1[0] = beta[s]
+            y, res, *_ = np.linalg.lstsq(Hbar[s, :j + 2, :j + 1], e1, rcond=None)
+            r_ls = np.linalg.norm(e1 - Hbar[s, :j + 2, :j + 1] @ y)
+            assert abs(abs(g[s, j + 1]) - r_ls) <= 1e-12 * beta[s]
+            R = np.triu(H[s, :j + 1, :j + 1])
+            yr = np.linalg.solve(R, g[s, :j + 1])
+            assert np.linalg.norm(yr - y) <= 1e-9 * max(1.0, np.linalg.norm(y))
+            js[s] += 1
+    # zero column and zero diagonal: identity rotation, and the swap rotation
+    cs1, sn1 = np.zeros((2, 2)), np.zeros((2, 2), np.complex128)
+    H1, g1 = np.zeros((2, 3, 2), np.complex128), np.zeros((2, 3), np.complex128)
+    g1[:, 0] = 1.0
+    _givens(gila, 2, 2, np.zeros(2, np.int64), np.zeros((2, 1), np.complex128), np.array([0.0, 2.0]),
+            cs1, sn1, H1, g1)
+    assert cs1[0, 0] == 1 and sn1[0, 0] == 0 and g1[0, 1] == 0
+    assert cs1[1, 0] == 0 and sn1[1, 0] == 1 and H1[1, 0, 0] == 2 and abs(g1[1, 1]) == 1
+    with pytest.raises(Exception):
+        _givens(gila, 1, 2, np.array([2], np.int64), np.zeros((1, 3), np.complex128), np.zeros(1),
+                cs1, sn1, H1, g1)
