@@ -146,7 +146,17 @@ class KrylovOps:
 
     def next(self, V, k, idx, w, w2, stream, stage=None):
         """V[r, idx[r]] = w[r] / max(sqrt(w2[r]), 1e-30) for r < k (w2 the
-        float64 device squared norms), also into stage[r] when given."""
+        float64 device squared norms), also into stage[r] when given.  On the
+        CPU (tests of the host logic) the same in torch."""
+        if not V.is_cuda:
+            torch = _torch()
+            inv = 1.0 / torch.clamp(torch.sqrt(w2[:k]), min=1e-30)
+            ar = torch.arange(k)
+            v = (w[:k].to(torch.complex128) * inv[:, None]).to(V.dtype)
+            V[ar, idx[:k]] = v
+            if stage is not None:
+                stage[:k] = v
+            return
         n = V.shape[-1]
         capi.check(capi.lib().gla_krylov_next(self._code(V), k, n, self._ptr(V), V.stride(0), V.stride(1),
                                               self._ptr(idx), self._ptr(w), w.stride(0), self._ptr(w2),
@@ -191,6 +201,12 @@ def gs_pass(kry, V, k, nv, w, stream, ncol=None, out=None):
         h = torch.einsum("rin,rn->ri", Vj.conj(), w64)
         w[:k] -= torch.einsum("ri,rin->rn", h, Vj).to(w.dtype)
         n1 = (w[:k].to(torch.complex128).abs() ** 2).sum(1)
+        if out is not None:
+            hv = out[:2 * k * nv].view(torch.complex128).view(k, nv)
+            hv.copy_(h)
+            out[2 * k * nv:2 * k * nv + k].copy_(n0)
+            out[2 * k * nv + k:2 * k * nv + 2 * k].copy_(n1)
+            return hv, out[2 * k * nv:2 * k * nv + k], out[2 * k * nv + k:2 * k * nv + 2 * k]
         return h, n0, n1
     if out is None:
         out = torch.empty(k * (2 * nv + 2), dtype=torch.float64, device=V.device)

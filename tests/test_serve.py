@@ -73,8 +73,8 @@ def test_server_solves_mixed_grids_on_cpu(gila, orc):
 
 
 def test_wide_batch_uses_vector_rotations(gila, orc):
-    # six requests on one grid: the Givens update runs over the batch (numpy)
-    # rather than per slot; results equal those solved alone
+    # six requests on one grid: one Givens update call covers the batch;
+    # results equal those solved alone
     reg = {}
     reqs = [gila.serve.random_request(500 + i, (4, 4, 4), tol=1e-9) for i in range(6)]
     _cpu_server(gila, orc, reg, mem_budget=1 << 30, max_batch=8).run(reqs)
@@ -84,6 +84,26 @@ def test_wide_batch_uses_vector_rotations(gila, orc):
         alone = gila.serve.random_request(500 + i, (4, 4, 4), tol=1e-9)
         _cpu_server(gila, orc, {}, mem_budget=1 << 30).run([alone])
         assert np.linalg.norm(alone.result - r.result) <= 1e-8 * np.linalg.norm(r.result)
+
+
+def test_queued_steps_match_one_step_at_a_time(gila, orc):
+    """Two Arnoldi steps queued per grid (the default) give the results and
+    step counts of one step at a time: requests of a batch converge at
+    different steps, and the step queued behind the one where a request's
+    cycle ends is discarded for that request."""
+    cells = [(4, 4, 4)] * 4 + [(6, 4, 4)] * 2
+    make = lambda: [gila.serve.random_request(300 + i, c, tol=1e-9) for i, c in enumerate(cells)]
+    one, two = make(), make()
+    _cpu_server(gila, orc, {}, mem_budget=1 << 30, max_batch=4, depth=1).run(one)
+    srv = _cpu_server(gila, orc, {}, mem_budget=1 << 30, max_batch=4, depth=2)
+    srv.run(two)
+    assert srv.ahead_steps > 0 and srv.discarded_steps > 0
+    for a, b in zip(one, two):
+        assert a.converged and b.converged
+        assert (a.steps, a.cycles) == (b.steps, b.cycles)
+        assert np.linalg.norm(a.result - b.result) <= 1e-12 * np.linalg.norm(a.result)
+    with pytest.raises(ValueError):
+        gila.serve.Server(device="cpu", depth=0)
 
 
 def test_alone_and_in_the_mix_agree(gila, orc):
