@@ -116,10 +116,10 @@ struct LenK {
     static constexpr int TP_ROW = N / pick_e(N, C::E_ROW, 64, true);
     static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
     // axis-2 passes: the bundle width, capped so the two parities' LDS
-    // buffers stay within GLA_YLDS_KB
+    // buffers and the staged tables stay within GLA_YLDS_KB
     static constexpr int PXY = [] {
         int px = C::PX;
-        while (px > 1 && 2 * px * N * (int)sizeof(cpx<R>) > GLA_YLDS_KB * 1024) px /= 2;
+        while (px > 1 && (2 * px + (GLA_XYTAB ? 2 : 0)) * N * (int)sizeof(cpx<R>) > GLA_YLDS_KB * 1024) px /= 2;
         return px;
     }();
     static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / PXY);

@@ -23,11 +23,14 @@ constexpr ZShape ztune(int complex_bytes, int n)
         switch (n) {
             // BEGIN TUNED C64
             case 16: return {4, 16};
-            case 20: return {4, 16};
+            case 20: return {4, 8};
             case 24: return {4, 16};
+            case 32: return {4, 8};
             case 40: return {4, 16};
             case 48: return {4, 16};
             case 80: return {4, 8};
+            case 96: return {4, 8};
+            case 160: return {4, 16};
             case 192: return {8, 8};
             case 256: return {8, 8};
             case 320: return {4, 8};
@@ -44,6 +47,8 @@ constexpr ZShape ztune(int complex_bytes, int n)
 #else
     switch (n) {
         // BEGIN TUNED C128
+            case 20: return {2, 8};
+            case 64: return {4, 4};
             case 160: return {4, 4};
             case 192: return {4, 4};
             case 256: return {8, 4};
