@@ -27,6 +27,20 @@
 #ifndef GLA_XYTAB
 #define GLA_XYTAB 1
 #endif
+// passes 2 and 4: the two parities through the transform together (2, one
+// LDS buffer each) or one after the other through one buffer (1: half the
+// LDS per workgroup, so more workgroups per CU)
+#ifndef GLA_YK
+#define GLA_YK 2
+#endif
+#ifndef GLA_YK_F
+#define GLA_YK_F GLA_YK
+#endif
+#ifndef GLA_YK_D
+#define GLA_YK_D GLA_YK
+#endif
+template <typename R> struct YK { static constexpr int value = GLA_YK_F; };
+template <> struct YK<double> { static constexpr int value = GLA_YK_D; };
 
 namespace gla {
 
@@ -349,7 +363,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restric
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[2 * PX * N];
+    __shared__ cpx<R> lds[YK<R>::value * PX * N];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PX);
@@ -372,7 +386,12 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restric
         v[0][e] = a + b;
         v[1][e] = phz[n] * (a - b);
     }
-    PencilFft<R, N, TP, PX, 1, false, 0, 2, PX * N>::run(v, lds, t, xl, tw);
+    if constexpr (YK<R>::value == 2) {
+        PencilFft<R, N, TP, PX, 1, false, 0, 2, PX * N>::run(v, lds, t, xl, tw);
+    } else {
+        PencilFft<R, N, TP, PX, 1, false>::run(v[0], lds, t, xl, tw);
+        PencilFft<R, N, TP, PX, 1, false>::run(v[1], lds, t, xl, tw);
+    }
     if (act) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -392,7 +411,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[2 * PX * N];
+    __shared__ cpx<R> lds[YK<R>::value * PX * N];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PX);
@@ -412,7 +431,12 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
         v[0][e] = act ? src[(int64_t)k * L1] : cpx<R>{R(0), R(0)};
         v[1][e] = act ? src[(int64_t)(N + k) * L1] : cpx<R>{R(0), R(0)};
     }
-    PencilFft<R, N, TP, PX, 1, true, 0, 2, PX * N>::run(v, lds, t, xl, tw);
+    if constexpr (YK<R>::value == 2) {
+        PencilFft<R, N, TP, PX, 1, true, 0, 2, PX * N>::run(v, lds, t, xl, tw);
+    } else {
+        PencilFft<R, N, TP, PX, 1, true>::run(v[0], lds, t, xl, tw);
+        PencilFft<R, N, TP, PX, 1, true>::run(v[1], lds, t, xl, tw);
+    }
     if (!act) return;
     const int top = T2 < N ? T2 : N;
 #pragma unroll

@@ -7,6 +7,12 @@
 #ifndef GLA_EROW
 #define GLA_EROW 8
 #endif
+#ifndef GLA_EROW_F
+#define GLA_EROW_F GLA_EROW
+#endif
+#ifndef GLA_EROW_D
+#define GLA_EROW_D GLA_EROW
+#endif
 // pass-3 schedule: 0 keeps the input column in registers through the even
 // parity, 2 splits both parities first (fewer registers for FP64)
 #ifndef GLA_ZRELOAD_F
@@ -103,7 +109,7 @@ template <typename R>
 struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
     static constexpr int PX = DBL ? GLA_PXY_D : GLA_PXY_F;   // axis-2 passes: columns per bundle
-    static constexpr int E_ROW = GLA_EROW;            // passes 1, 2, 4, 5
+    static constexpr int E_ROW = DBL ? GLA_EROW_D : GLA_EROW_F;   // passes 1, 2, 4, 5
     static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
     static constexpr int ZMINW = DBL ? GLA_ZMINW_D : GLA_ZMINW_F;
     static constexpr int ZGG = DBL ? GLA_ZGG_D : GLA_ZGG_F;
@@ -119,7 +125,7 @@ struct LenK {
     // buffers and the staged tables stay within GLA_YLDS_KB
     static constexpr int PXY = [] {
         int px = C::PX;
-        while (px > 1 && (2 * px + (GLA_XYTAB ? 2 : 0)) * N * (int)sizeof(cpx<R>) > GLA_YLDS_KB * 1024) px /= 2;
+        while (px > 1 && (YK<R>::value * px + (GLA_XYTAB ? 2 : 0)) * N * (int)sizeof(cpx<R>) > GLA_YLDS_KB * 1024) px /= 2;
         return px;
     }();
     static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / PXY);
