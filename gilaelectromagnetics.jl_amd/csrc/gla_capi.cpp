@@ -668,6 +668,7 @@ gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, i
 namespace {
 gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, void* stream, int32_t reps, double* ms,
                           int64_t* bytes);
+void pass_bytes(gla_opr_t h, int64_t nrhs, int64_t* bytes);
 }
 extern "C" {
 
@@ -683,15 +684,59 @@ gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int6
     return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);
 }
 
+gla_status gla_opr_profile_seq(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
+                               void* stream, int32_t reps, double* ms, int64_t* bytes, double* total_ms)
+{
+    if (!h || !ms || !bytes || !total_ms || reps < 1 || reps > 100000) return fail(GLA_ERR_ARG, "bad profile arguments");
+    gla_status s = run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);   // validates, allocates, warms up
+    if (s) return s;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DevGuard g(h->opts.device);
+    pass_bytes(h, nrhs, bytes);   // algorithmic bytes per pass, as gla_opr_profile
+    // reps matvecs exactly as gla_opr_apply issues them, with an event at
+    // every pass boundary: pass p of matvec r lasts ev[r][p] -> ev[r][p+1]
+    const int ne = gla::NPASS + 1;
+    std::vector<hipEvent_t> ev((size_t)reps * ne, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto& v : ev)
+        if (e == hipSuccess) e = hipEventCreate(&v);
+    hipStream_t st = (hipStream_t)stream;
+    const gla::Geom& G = h->geom;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        e = hipEventRecord(ev[(size_t)r * ne], st);
+        for (int p = 0; p < gla::NPASS && e == hipSuccess; ++p) {
+            e = h->dims.dtype == GLA_C64
+                    ? gla::run_pass<float>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(), st)
+                    : gla::run_pass<double>(p, G, h->tab, x, y, h->w1, h->w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(),
+                                            st);
+            if (e == hipSuccess) e = hipEventRecord(ev[(size_t)r * ne + p + 1], st);
+        }
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(ev.back());
+    double acc[gla::NPASS] = {0, 0, 0, 0, 0};
+    for (int r = 0; r < reps && e == hipSuccess; ++r)
+        for (int p = 0; p < gla::NPASS && e == hipSuccess; ++p) {
+            float t = 0;
+            e = hipEventElapsedTime(&t, ev[(size_t)r * ne + p], ev[(size_t)r * ne + p + 1]);
+            acc[p] += t;
+        }
+    float tot = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&tot, ev.front(), ev.back());
+    for (auto& v : ev)
+        if (v) (void)hipEventDestroy(v);
+    if (e != hipSuccess) return fail(GLA_ERR_HIP, std::string("profile: ") + hipGetErrorString(e));
+    for (int p = 0; p < gla::NPASS; ++p) ms[p] = acc[p] / reps;
+    *total_ms = (double)tot / reps;
+    return GLA_OK;
+}
+
 }  // extern "C"
 
 namespace {
 
-gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, void* stream, int32_t reps, double* ms,
-                          int64_t* bytes)
+// algorithmic HBM bytes of each pass of one matvec over nrhs right-hand sides
+void pass_bytes(gla_opr_t h, int64_t nrhs, int64_t* bytes)
 {
-    std::lock_guard<std::mutex> lk(h->mu);
-    DevGuard g(h->opts.device);
     const gla::Geom& G = h->geom;
     const int64_t eb = (int64_t)elem_bytes(h->dims.dtype);
     const int64_t S = (int64_t)G.S[0] * G.S[1] * G.S[2], T = (int64_t)G.T[0] * G.T[1] * G.T[2];
@@ -709,6 +754,15 @@ gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, voi
     bytes[2] = f * (a2 + a3) + gsz * eb;
     bytes[3] = f * (a3 + a4);
     bytes[4] = f * (a4 + T);
+}
+
+gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, void* stream, int32_t reps, double* ms,
+                          int64_t* bytes)
+{
+    std::lock_guard<std::mutex> lk(h->mu);
+    DevGuard g(h->opts.device);
+    const gla::Geom& G = h->geom;
+    pass_bytes(h, nrhs, bytes);
     hipEvent_t e0, e1;
     GLA_HIP(hipEventCreate(&e0));
     GLA_HIP(hipEventCreate(&e1));

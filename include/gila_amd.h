@@ -176,6 +176,15 @@ gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, i
 gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, void* stream,
                            int32_t reps, double* ms, int64_t* bytes);
 
+/* The same figures measured inside the real sequence: `reps` matvecs issued
+ * exactly as gla_opr_apply issues them, with an event at every pass
+ * boundary; ms[p] is pass p's average duration between its boundary events,
+ * bytes[p] its algorithmic HBM bytes and *total_ms the average duration of
+ * one whole matvec over the run.  Synchronizes; y holds a valid matvec on
+ * return.  No counterpart in the reference; for benchmarks. */
+gla_status gla_opr_profile_seq(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
+                               void* stream, int32_t reps, double* ms, int64_t* bytes, double* total_ms);
+
 /* The operator's target and source volumes, and its options, as created. */
 gla_status gla_opr_get_vols(gla_opr_t h, gla_vol* trg, gla_vol* src);
 gla_status gla_opr_get_opts(gla_opr_t h, gla_opts* opts);
