@@ -208,6 +208,23 @@ def test_profile_entry_leaves_a_valid_matvec(gila, orc, gpu):
     assert all(ms[i] > 0 for i in range(5)) and all(by[i] > 0 for i in range(5))
     y = yd.cpu().numpy().reshape(N + (3,), order="F")
     assert orc.rel_l2(y, orc.matvec(N, N, (1, 1, 1), fur, x)) <= 1e-5
+    # the in-sequence form: the same byte figures, positive pass durations
+    # whose sum fits in the whole-matvec time, and y bitwise the plain apply
+    ms2 = (ctypes.c_double * 5)()
+    by2 = (ctypes.c_int64 * 5)()
+    tot = ctypes.c_double()
+    ys = torch.empty_like(xd)
+    _capi.check(gila.lib().gla_opr_profile_seq(mem._h, ctypes.c_void_p(xd.data_ptr()), xd.numel(),
+                                               ctypes.c_void_p(ys.data_ptr()), ys.numel(), 1, ctypes.c_void_p(st), 4,
+                                               ms2, by2, ctypes.byref(tot)))
+    torch.cuda.synchronize()
+    assert [by2[i] for i in range(5)] == [by[i] for i in range(5)]
+    assert all(ms2[i] > 0 for i in range(5)) and sum(ms2[i] for i in range(5)) <= tot.value * 1.001
+    assert torch.equal(ys.cpu(), yd.cpu())
+    with pytest.raises(gila.GlaError):
+        _capi.check(gila.lib().gla_opr_profile_seq(mem._h, ctypes.c_void_p(xd.data_ptr()), xd.numel(),
+                                                   ctypes.c_void_p(ys.data_ptr()), ys.numel(), 1,
+                                                   ctypes.c_void_p(st), 0, ms2, by2, ctypes.byref(tot)))
 
 
 @pytest.mark.gpu
