@@ -156,3 +156,24 @@ def test_server_on_gpu_matches_fp64_oracle_residual(gila, orc, gpu):
     target = gila.serve.random_request(400, (16, 16, 16), tol=1e-6)
     alone = gila.serve.Server(max_batch=4, mem_budget=8 << 30).run([target], timeout=60)[0]
     assert np.linalg.norm(alone.result - reqs[0].result) <= 1e-5 * np.linalg.norm(alone.result)
+
+
+@pytest.mark.gpu
+def test_server_queued_steps_on_gpu(gila, orc, gpu):
+    """Two Arnoldi steps queued per grid on the GPU: the same step and cycle
+    counts as one step at a time, results within rounding of each other and
+    of the FP64 oracle residual."""
+    print("case serve gpu depth 2", flush=True)
+    cells_list = [(16, 16, 16), (16, 16, 16), (16, 16, 16), (24, 16, 8)]
+    make = lambda: [gila.serve.random_request(450 + i, c, tol=1e-6) for i, c in enumerate(cells_list)]
+    one, two = make(), make()
+    gila.serve.Server(max_batch=4, mem_budget=8 << 30, depth=1).run(one, timeout=120)
+    srv = gila.serve.Server(max_batch=4, mem_budget=8 << 30, depth=2)
+    srv.run(two, timeout=120)
+    assert srv.ahead_steps > 0
+    for a, b in zip(one, two):
+        assert a.converged and b.converged
+        assert (a.steps, a.cycles) == (b.steps, b.cycles)
+        assert np.linalg.norm(a.result - b.result) <= 1e-6 * np.linalg.norm(a.result)
+        fur = np.concatenate([f.reshape(-1, order="F") for f in srv.ops[b.key].mem64.egoFur])
+        assert _residual(orc, fur, b) <= 2e-6
