@@ -99,13 +99,21 @@ struct gla_opr_s {
     void* w1 = nullptr;
     void* w2 = nullptr;
     int64_t ws_nrhs = 0;
+    // device copies of the host arrays of gla_opr_apply_host (grown as
+    // needed, kept until release), under their own lock
+    void* hx = nullptr;
+    void* hy = nullptr;
+    int64_t hx_n = 0, hy_n = 0;
+    std::mutex hmu;
     std::mutex mu;
     ~gla_opr_s()
     {
-        if (w1 || w2) {
+        if (w1 || w2 || hx || hy) {
             DevGuard g(opts.device);
             if (w1) (void)hipFree(w1);
             if (w2) (void)hipFree(w2);
+            if (hx) (void)hipFree(hx);
+            if (hy) (void)hipFree(hy);
         }
     }
 };
@@ -643,22 +651,29 @@ gla_status gla_ls_apply(gla_opr_t h, const void* chi, int64_t nchi, int64_t chi_
 gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs)
 {
     if (!h) return fail(GLA_ERR_ARG, "null operator");
+    if (nrhs < 1 || nx != h->geom.xs * nrhs || ny != h->geom.ys * nrhs || !x || !y)
+        return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, nullptr);   // raises the argument or size error
+    std::lock_guard<std::mutex> lk(h->hmu);
     DevGuard g(h->opts.device);
     const size_t eb = elem_bytes(h->dims.dtype);
-    void *dx = nullptr, *dy = nullptr;
-    if (nx != h->geom.xs * nrhs || ny != h->geom.ys * nrhs)
-        return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, nullptr);   // raises the size error
-    GLA_HIP(hipMalloc(&dx, (size_t)nx * eb));
-    hipError_t e = hipMalloc(&dy, (size_t)ny * eb);
-    if (e != hipSuccess) { (void)hipFree(dx); return fail(GLA_ERR_HIP, hipGetErrorString(e)); }
+    auto grow = [&](void*& p, int64_t& have, int64_t want) -> hipError_t {
+        if (have >= want) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        have = 0;
+        const hipError_t e = hipMalloc(&p, (size_t)want * eb);
+        if (e == hipSuccess) have = want;
+        return e;
+    };
+    hipError_t e = grow(h->hx, h->hx_n, nx);
+    if (e == hipSuccess) e = grow(h->hy, h->hy_n, ny);
+    if (e != hipSuccess) return fail(GLA_ERR_HIP, hipGetErrorString(e));
     gla_status s = GLA_OK;
-    e = hipMemcpy(dx, x, (size_t)nx * eb, hipMemcpyHostToDevice);
+    e = hipMemcpy(h->hx, x, (size_t)nx * eb, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        s = run(h, dx, nx, dy, ny, nrhs, 1, 0, 0, 0, 0, nullptr);
-        if (s == GLA_OK) e = hipMemcpy(y, dy, (size_t)ny * eb, hipMemcpyDeviceToHost);
+        s = run(h, h->hx, nx, h->hy, ny, nrhs, 1, 0, 0, 0, 0, nullptr);
+        if (s == GLA_OK) e = hipMemcpy(y, h->hy, (size_t)ny * eb, hipMemcpyDeviceToHost);
     }
-    (void)hipFree(dx);
-    (void)hipFree(dy);
     if (s) return s;
     if (e != hipSuccess) return fail(GLA_ERR_HIP, hipGetErrorString(e));
     return GLA_OK;

@@ -162,7 +162,9 @@ gla_status gla_ls_apply(gla_opr_t h, const void* chi, int64_t nchi, int64_t chi_
                         void* y, int64_t ny, int64_t nrhs, void* stream);
 
 /* G * x on host memory (src/glaLinAlg.jl:33-43 with the host-to-device copy
- * of :37-40): copies in, applies, copies out and synchronizes. */
+ * of :37-40): copies in, applies, copies out and synchronizes.  The device
+ * copies of x and y are kept by the handle (grown to the largest call) until
+ * gla_opr_release. */
 gla_status gla_opr_apply_host(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs);
 
 /* Per-kernel timing of the matvec with HIP events on `stream`: after one
