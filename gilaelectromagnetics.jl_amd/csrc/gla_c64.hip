@@ -4,4 +4,3 @@ template hipError_t gla::run_matvec<float>(const gla::Geom&, const gla::DevTab&,
                                            int, double, double, double, double, int, const gla::Epi&, hipStream_t);
 template hipError_t gla::run_pass<float>(int, const gla::Geom&, const gla::DevTab&, const void*, void*, void*, void*,
                                       int, double, double, double, double, int, const gla::Epi&, hipStream_t);
-bool gla::xy_fused_f32(const gla::Geom& g) { return gla::xy_fused<float>(g); }

@@ -761,11 +761,6 @@ void pass_bytes(gla_opr_t h, int64_t nrhs, int64_t* bytes)
     const int64_t f = 3 * nrhs * eb;
     bytes[0] = f * (S + a1);
     bytes[1] = f * (a1 + a2);
-    if (h->dims.dtype == GLA_C64 && gla::xy_fused_f32(G)) {
-        // passes 1 and 2 run as one kernel (reported as pass 1): X in, W2 out
-        bytes[0] = f * (S + a2);
-        bytes[1] = 0;
-    }
     bytes[2] = f * (a2 + a3) + gsz * eb;
     bytes[3] = f * (a3 + a4);
     bytes[4] = f * (a4 + T);

@@ -201,98 +201,6 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
 }
 
 // --------------------------------------------------------------------------
-// Passes 1 and 2 fused (B1 = B2 = N): one workgroup per (plane (i3, c), x
-// parity p1, right-hand side).  Phase 1 transforms the plane's rows (x split
-// for parity p1, length-N FFT) into an LDS tile, row i2 at tile[i2*RS ..];
-// each row pencil uses its own tile row as exchange space.  Phase 2 reads the
-// tile's columns (y split into both parities, two length-N FFTs, each column
-// pencil using its own column slots as exchange space) and writes W2
-// directly: the intermediate W1 of the two-pass form never goes to HBM.
-// Rows beyond S2 are scratch for masked pencils; the tile holds
-// XY_ROWS(N) >= max(S2 rounded to the row-pencil count, N) rows.
-// --------------------------------------------------------------------------
-template <int N>
-struct XyTile {
-    static constexpr int RS = (N + (N >> 3) + 1) | 1;   // odd pitch: column accesses spread over banks
-    static constexpr int SLOTS = 20480;                  // 160 KB of ComplexF32
-    static constexpr int ROWS = SLOTS / RS;
-};
-
-template <typename R, int N, int TP1, int TP2, int NT>
-__global__ __launch_bounds__(NT, 1) void k_fwd_xy(const cpx<R>* __restrict__ X, cpx<R>* __restrict__ W2, int S1,
-                                                  int S2, int S3, int L1, int Z, int64_t xs, int64_t w2s,
-                                                  const cpx<R>* __restrict__ phz1, const cpx<R>* __restrict__ tw1,
-                                                  const cpx<R>* __restrict__ phz2, const cpx<R>* __restrict__ tw2)
-{
-    using T = XyTile<N>;
-    constexpr int RS = T::RS;
-    constexpr int E1 = N / TP1, E2 = N / TP2;
-    constexpr int PEN1 = NT / TP1, CPB = NT / TP2;
-    static_assert(64 % TP1 == 0 && 64 % TP2 == 0 && NT % 64 == 0, "pencils inside one wave");
-    static_assert(N % CPB == 0, "whole column groups");
-    static_assert(T::ROWS >= N, "the tile holds N rows");
-    __shared__ cpx<R> tile[T::SLOTS];
-    const int pl = blockIdx.x;
-    const int c = pl / S3, i3 = pl - c * S3;
-    const int p1 = blockIdx.y;
-    const int64_t rhs = blockIdx.z;
-
-    // phase 1: rows i2 of the plane, x split for parity p1 and x-FFT
-    {
-        const int t = threadIdx.x % TP1, pen = threadIdx.x / TP1;
-        const cpx<R>* xp = X + rhs * xs + ((int64_t)c * S3 + i3) * S2 * (int64_t)S1;
-        for (int r0 = 0; r0 < S2; r0 += PEN1) {
-            const int i2 = r0 + pen;
-            const bool act = i2 < S2;
-            const cpx<R>* xr = xp + (int64_t)(act ? i2 : 0) * S1;
-            cpx<R> v[1][E1];
-#pragma unroll
-            for (int e = 0; e < E1; ++e) {
-                const int n = t + TP1 * e;
-                cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-                if (act && n < S1) a = xr[n];
-                if (act && n + N < S1) b = xr[n + N];
-                v[0][e] = p1 == 0 ? a + b : phz1[n] * (a - b);
-            }
-            PencilFft<R, N, TP1, 1, RS, false, 1, 1, 0, 3>::run(v, tile, t, i2, tw1);
-#pragma unroll
-            for (int e = 0; e < E1; ++e) tile[i2 * RS + t + TP1 * e] = v[0][e];
-        }
-    }
-    __syncthreads();
-
-    // phase 2: columns k1 of the tile, y split into both parities and y-FFTs
-    {
-        const int t = threadIdx.x % TP2, pen = threadIdx.x / TP2;
-        cpx<R>* dst = W2 + rhs * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + (int64_t)p1 * N;
-        for (int k0 = 0; k0 < N; k0 += CPB) {
-            const int k1 = k0 + pen;
-            cpx<R> a[E2], b[E2];
-#pragma unroll
-            for (int e = 0; e < E2; ++e) {
-                const int n = t + TP2 * e;
-                a[e] = n < S2 ? tile[n * RS + k1] : cpx<R>{R(0), R(0)};
-                b[e] = n + N < S2 ? tile[(n + N) * RS + k1] : cpx<R>{R(0), R(0)};
-            }
-            lds_sync<1>();   // the column's reads are done before its slots become exchange space
-#pragma unroll
-            for (int p2 = 0; p2 < 2; ++p2) {
-                cpx<R> v[1][E2];
-#pragma unroll
-                for (int e = 0; e < E2; ++e) {
-                    const int n = t + TP2 * e;
-                    v[0][e] = p2 == 0 ? a[e] + b[e] : phz2[n] * (a[e] - b[e]);
-                }
-                PencilFft<R, N, TP2, RS, 1, false, 1, 1, 0, 0>::run(v, tile, t, k1, tw2);
-#pragma unroll
-                for (int e = 0; e < E2; ++e)
-                    dst[(int64_t)(p2 * N + t + TP2 * e) * L1 + k1] = v[0][e];
-            }
-        }
-    }
-}
-
-// --------------------------------------------------------------------------
 // Pass 5: x-inverse + merge/crop onto T1, optional y = alpha*res + beta*y.
 // --------------------------------------------------------------------------
 template <typename R, int N, int TP, int PEN, int MINW>
@@ -679,144 +587,6 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
             if constexpr (MODE == 2)
                 if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = accb[c][e];
         }
-}
-
-// --------------------------------------------------------------------------
-// Pass 3, software-pipelined: a persistent form of k_zpass for S3 <= B3.
-// Each workgroup walks its XCD's part of the tile order (tiles l = xcd *
-// per_xcd + q + i * bpx for its slot q of bpx per XCD) and loads the next
-// tile's columns while the current one is transformed, so the memory phase of
-// one tile overlaps the arithmetic of the other.  The next tile's loads are
-// issued after the even parity's G^ multiply: vector-memory completions are
-// counted in order, so issuing them earlier would make the even parity's G^
-// loads wait for them.  Same arithmetic as k_zpass with both parities split
-// first.
-// --------------------------------------------------------------------------
-template <typename R, int N, int TP, int PX, int MODE, int KB, int PF>
-__global__ __launch_bounds__(TP * PX, 2) void k_zpipe(cpx<R>* __restrict__ W2, int L1, int L2, int S3, int T3,
-                                                   int Z, int64_t w2s, const cpx<R>* __restrict__ gf,
-                                                   AxisMap a1, AxisMap a2, AxisMap a3,
-                                                   const cpx<R>* __restrict__ phz,
-                                                   const cpx<R>* __restrict__ tw,
-                                                   const int* __restrict__ order, int per_xcd, int nrhs)
-{
-    static_assert(MODE != 1, "the pipelined pass needs S3 <= B3");
-    (void)nrhs;   // right-hand side from blockIdx.z in this form
-    constexpr int E = N / TP;
-    __shared__ cpx<R> lds[KB * PX * N];
-    int xl = threadIdx.x % PX;
-    int t = threadIdx.x / PX;
-    const int xcd = blockIdx.x % 8, bpx = gridDim.x / 8;
-    int il = blockIdx.x / 8;
-    if (il >= per_xcd) return;   // whole workgroup
-    const int64_t cs = (int64_t)Z * L2 * L1;
-    const int64_t zs = (int64_t)L2 * L1;
-    cpx<R>* const base = W2 + (int64_t)blockIdx.z * w2s;
-    const int uu = a1.U * a2.U;
-
-    auto fwd = [&](cpx<R>(&w)[3][E]) {
-        if constexpr (KB == 3) {
-            PencilFft<R, N, TP, PX, 1, false, 0, 3, PX * N>::run(w, lds, t, xl, tw);
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, false>::run(w[c], lds, t, xl, tw);
-        }
-    };
-    auto inv = [&](cpx<R>(&w)[3][E]) {
-        if constexpr (KB == 3) {
-            PencilFft<R, N, TP, PX, 1, true, 0, 3, PX * N>::run(w, lds, t, xl, tw);
-        } else {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) PencilFft<R, N, TP, PX, 1, true>::run(w[c], lds, t, xl, tw);
-        }
-    };
-    // this thread's column of a tile (order entry tile | m2 << 16, -1 padding)
-    auto column = [&](int tile, int& m1, int& m2) {
-        m1 = (tile & 0xffff) * PX + xl;
-        m2 = tile >> 16;
-        return tile >= 0 && m1 < L1;
-    };
-    // components [c0, c1) of a tile's columns
-    auto load = [&](int tile, cpx<R>(&xa)[3][E], int c0, int c1) {
-        int m1, m2;
-        const bool ok = column(tile, m1, m2);
-        const cpx<R>* col = base + (ok ? (int64_t)m2 * L1 + m1 : 0);
-#pragma unroll
-        for (int c = c0; c < c1; ++c)
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int n = t + TP * e;
-                xa[c][e] = (ok && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
-            }
-    };
-
-    int tile = order[xcd * per_xcd + il];
-    cpx<R> xa[3][E];
-    load(tile, xa, 0, 3);
-    for (;;) {
-        // table pointers opaque per tile: keeps the twiddle and phase loads
-        // inside the loop instead of holding their values in registers
-        // across all tiles
-        // the thread index opaque per tile: what derives from it (offsets,
-        // phases, mirror lookups) is recomputed per tile rather than held in
-        // registers across the loop
-        asm volatile("" : "+v"(t), "+v"(xl));
-        int m1, m2;
-        const bool act = column(tile, m1, m2);
-        const int p1 = (act && m1 >= a1.B) ? 1 : 0, k1 = act ? m1 - p1 * a1.B : 0;
-        const int p2 = (act && m2 >= a2.B) ? 1 : 0, k2 = act ? m2 - p2 * a2.B : 0;
-        int j1, s1, j2, s2;
-        axis_lookup(a1, p1, k1, j1, s1);
-        axis_lookup(a2, p2, k2, j2, s2);
-        const cpx<R>* gc = gf + (int64_t)(j2 * a1.U + j1) * 6;
-        cpx<R> wv[2][3][E];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int n = t + TP * e;
-                wv[0][c][e] = xa[c][e];
-                wv[1][c][e] = phz[n] * xa[c][e];
-            }
-        const int nil = il + bpx;
-        const bool more = nil < per_xcd;   // the same for the whole workgroup
-        const int ntile = more ? order[xcd * per_xcd + nil] : -1;
-        const int b0 = (p1 << 2) | (p2 << 1);
-        fwd(wv[0]);
-        g_multiply<R, E, TP>(wv[0], gc + (int64_t)b0 * a3.U * uu * 6, uu, a3, 0, t, s1, s2, act);
-        // the next tile's first PF components load during the rest of this
-        // tile's transforms, the others during the odd parity's inverse
-        __builtin_amdgcn_sched_barrier(0);
-        if (more && PF > 0) load(ntile, xa, 0, PF);
-        __builtin_amdgcn_sched_barrier(0);
-        inv(wv[0]);
-        fwd(wv[1]);
-        g_multiply<R, E, TP>(wv[1], gc + (int64_t)(b0 | 1) * a3.U * uu * 6, uu, a3, 1, t, s1, s2, act);
-        if constexpr (PF >= 0 && PF < 3) {
-            __builtin_amdgcn_sched_barrier(0);
-            if (more) load(ntile, xa, PF, 3);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        inv(wv[1]);
-        if (act) {
-            cpx<R>* col = base + (int64_t)m2 * L1 + m1;
-            const int top = T3 < N ? T3 : N;
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const int n = t + TP * e;
-                    const cpx<R> po = cmulc(wv[1][c][e], phz[n]);
-                    if (n < top) col[c * cs + (int64_t)n * zs] = wv[0][c][e] + po;
-                    if constexpr (MODE == 2)
-                        if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = wv[0][c][e] - po;
-                }
-        }
-        if (!more) break;
-        il = nil;
-        tile = ntile;
-        if constexpr (PF < 0) load(tile, xa, 0, 3);
-    }
 }
 
 // --------------------------------------------------------------------------

@@ -42,8 +42,6 @@ int zpass_tile_width(int complex_bytes, int n);
 // next to each other.
 std::vector<int> zpass_order(const int B[3], const int L[3], const AxisMap am[3], int px);
 
-// whether passes 1 and 2 run as one fused kernel for this geometry (ComplexF32)
-bool xy_fused_f32(const Geom& g);
 
 // last-pass epilogue operands for y = x - chi (G x) (axpby mode 2)
 struct Epi {

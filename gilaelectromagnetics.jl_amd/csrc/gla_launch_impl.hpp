@@ -61,19 +61,6 @@
 #ifndef GLA_ZPADSH
 #define GLA_ZPADSH 3
 #endif
-// pass 3 as a persistent, software-pipelined kernel when S3 <= B3 (off: at
-// 128^3 it measured 230-240 us against 211 us for k_zpass, the persistent
-// loop costing more registers than the overlap gains)
-#ifndef GLA_ZPIPE
-#define GLA_ZPIPE 0
-#endif
-// components of the next tile loaded early in the pipelined pass 3
-#ifndef GLA_ZPF_F
-#define GLA_ZPF_F 2
-#endif
-#ifndef GLA_ZPF_D
-#define GLA_ZPF_D 2
-#endif
 
 namespace gla {
 
@@ -196,33 +183,13 @@ static hipError_t launch_zpass_chunk(const Geom& g, const DevTab& tb, void* W2, 
     using K = LenK<R, N>;
     constexpr int PX = K::PXZ;
     constexpr int SC = Cfg<R>::ZSCHED;
-    constexpr int PF = sizeof(R) == 8 ? GLA_ZPF_D : GLA_ZPF_F;
     dim3 grid(8 * tb.zper_xcd * nrhs, 1, 1);   // right-hand side inside the XCD stretch
     dim3 blk(K::TP_Z * PX);
     const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);
 #define GLA_ZARGS                                                                                   \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
         tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd, nrhs
-    if (GLA_ZPIPE && mode != 1 && !K::WL_Z) {
-        // one resident wave of workgroups: blocks per CU from the occupancy
-        // of this kernel, 8 XCDs of CUs/8 each
-        static int bpx = 0;
-        if (bpx == 0) {
-            int dev = 0, ncu = 0, nb = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-                return hipErrorInvalidValue;
-            const hipError_t oe = mode == 0
-                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zpipe<R, N, K::TP_Z, PX, 0, K::KB_Z, PF>, K::TP_Z * PX, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_zpipe<R, N, K::TP_Z, PX, 2, K::KB_Z, PF>, K::TP_Z * PX, 0);
-            if (oe != hipSuccess) return oe;
-            bpx = (nb < 1 ? 1 : nb) * ((ncu + 7) / 8);
-        }
-        const int nb_xcd = bpx < tb.zper_xcd ? bpx : tb.zper_xcd;
-        dim3 pgrid(8 * nb_xcd, 1, nrhs);
-        if (mode == 0) hipLaunchKernelGGL((k_zpipe<R, N, K::TP_Z, PX, 0, K::KB_Z, PF>), pgrid, blk, 0, st, GLA_ZARGS);
-        else hipLaunchKernelGGL((k_zpipe<R, N, K::TP_Z, PX, 2, K::KB_Z, PF>), pgrid, blk, 0, st, GLA_ZARGS);
-    } else if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
     else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
     else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
 #undef GLA_ZARGS
@@ -256,75 +223,6 @@ static hipError_t launch_fwd_z(const Geom& g, const DevTab& tb, void* W2, int nr
     return hipGetLastError();
 }
 
-// Passes 1 and 2 fused (k_fwd_xy): ComplexF32, B1 = B2 = N in {32, 64, 96,
-// 128}, and the plane's rows fit the LDS tile.  Off: the 160 KB tile leaves
-// one workgroup per CU, whose load, transform and store phases then run back
-// to back -- measured 114 us against 29 + 71 us for the two passes at 128^3
-// (84 against 40 us at 96^3), with 1024 threads per workgroup 121 us.
-#ifndef GLA_XYFUSE
-#define GLA_XYFUSE 0
-#endif
-#ifndef GLA_XYNT
-#define GLA_XYNT 512
-#endif
-template <int N>
-struct XyK {
-    static constexpr int TP = N / pick_e(N, GLA_EROW, 64, true);
-    // columns per phase-2 group: GLA_XYNT threads when they divide the columns
-    static constexpr int CPB = (GLA_XYNT / TP <= N && N % (GLA_XYNT / TP) == 0) ? GLA_XYNT / TP
-                                                                                 : ((512 / TP <= N) ? 512 / TP : N);
-    static constexpr int NT = TP * CPB;
-    static constexpr int PEN1 = NT / TP;
-};
-
-template <int N>
-static bool xy_fits(const Geom& g)
-{
-    const int pen = XyK<N>::PEN1;
-    const int rows = ((g.S[1] + pen - 1) / pen) * pen;
-    return (rows > N ? rows : N) <= XyTile<N>::ROWS;
-}
-
-template <typename R>
-bool xy_fused(const Geom& g)
-{
-    if (!GLA_XYFUSE || sizeof(R) != 4 || g.B[0] != g.B[1]) return false;
-    switch (g.B[0]) {
-        case 32: return xy_fits<32>(g);
-        case 64: return xy_fits<64>(g);
-        case 96: return xy_fits<96>(g);
-        case 128: return xy_fits<128>(g);
-        default: return false;
-    }
-}
-
-template <int N>
-static hipError_t launch_fwd_xy(const Geom& g, const DevTab& tb, const void* X, void* W2, int nrhs, hipStream_t st)
-{
-    using K = XyK<N>;
-    dim3 grid(g.S[2] * 3, 2, nrhs);
-    hipLaunchKernelGGL((k_fwd_xy<float, N, K::TP, K::TP, K::NT>), grid, dim3(K::NT), 0, st, (const cpx<float>*)X,
-                       (cpx<float>*)W2, g.S[0], g.S[1], g.S[2], g.L[0], g.Z, g.xs, g.w2s,
-                       (const cpx<float>*)tb.phz[0], (const cpx<float>*)tb.tw[0], (const cpx<float>*)tb.phz[1],
-                       (const cpx<float>*)tb.tw[1]);
-    return hipGetLastError();
-}
-
-template <typename R>
-static hipError_t launch_fused_xy(const Geom& g, const DevTab& tb, const void* X, void* W2, int nrhs, hipStream_t st)
-{
-    if constexpr (sizeof(R) == 4) {
-        switch (g.B[0]) {
-            case 32: return launch_fwd_xy<32>(g, tb, X, W2, nrhs, st);
-            case 64: return launch_fwd_xy<64>(g, tb, X, W2, nrhs, st);
-            case 96: return launch_fwd_xy<96>(g, tb, X, W2, nrhs, st);
-            case 128: return launch_fwd_xy<128>(g, tb, X, W2, nrhs, st);
-            default: break;
-        }
-    }
-    return hipErrorInvalidValue;
-}
-
 template <typename R>
 hipError_t run_fwd3(const Geom& g, const DevTab& tb, const void* X, void* W1, void* W2, int nrhs, hipStream_t st)
 {
@@ -355,16 +253,9 @@ hipError_t run_pass(int pass, const Geom& g, const DevTab& tb, const void* X, vo
 #define GLA_CASE_3(n) case n: e = launch_zpass<R, n>(g, tb, W2, nrhs, st); break;
 #define GLA_CASE_4(n) case n: e = launch_inv_y<R, n>(g, tb, W2, W1, nrhs, st); break;
 #define GLA_CASE_5(n) case n: e = launch_inv_x<R, n>(g, tb, W1, Y, nrhs, al, be, axpby, epi, st); break;
-    const bool fused = xy_fused<R>(g);
     switch (pass) {
-        case 0:
-            if (fused) return launch_fused_xy<R>(g, tb, X, W2, nrhs, st);
-            switch (g.B[0]) { GLA_FOR_EACH_LEN(GLA_CASE_1) default: break; }
-            break;
-        case 1:
-            if (fused) return hipSuccess;   // done by the fused pass 1
-            switch (g.B[1]) { GLA_FOR_EACH_LEN(GLA_CASE_2) default: break; }
-            break;
+        case 0: switch (g.B[0]) { GLA_FOR_EACH_LEN(GLA_CASE_1) default: break; } break;
+        case 1: switch (g.B[1]) { GLA_FOR_EACH_LEN(GLA_CASE_2) default: break; } break;
         case 2: switch (g.B[2]) { GLA_FOR_EACH_LEN(GLA_CASE_3) default: break; } break;
         case 3: switch (g.B[1]) { GLA_FOR_EACH_LEN(GLA_CASE_4) default: break; } break;
         case 4: switch (g.B[0]) { GLA_FOR_EACH_LEN(GLA_CASE_5) default: break; } break;

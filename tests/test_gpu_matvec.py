@@ -266,17 +266,16 @@ def test_every_tuned_pass3_length(gila, orc, gpu, n, typ):
     assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(N, N, (1, 1, 1), fur, x)) <= TOL[typ]
 
 
-FUSED_SELF = [(32, 32, 8), (64, 64, 6), (96, 96, 4), (128, 128, 2)]
-FUSED_EXT = [((32, 40, 4), (32, 24, 4)), ((32, 24, 4), (32, 40, 4)), ((64, 70, 3), (64, 58, 5))]
+SQUARE_SELF = [(32, 32, 8), (64, 64, 6), (96, 96, 4), (128, 128, 2)]
+SQUARE_EXT = [((32, 40, 4), (32, 24, 4)), ((32, 24, 4), (32, 40, 4)), ((64, 70, 3), (64, 58, 5))]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", FUSED_SELF)
-def test_fused_xy_self(gila, orc, gpu, N):
-    # ComplexF32 with B1 = B2 in {32, 64, 96, 128}: the shapes the fused
-    # pass-1+2 kernel covers when the library is built with it (GLA_XYFUSE=1;
-    # the default build runs them through the two passes)
-    print(f"case fused xy self N={N}", flush=True)
+@pytest.mark.parametrize("N", SQUARE_SELF)
+def test_square_xy_planes_self(gila, orc, gpu, N):
+    # ComplexF32 with B1 = B2 in {32, 64, 96, 128}, thin along z, two
+    # right-hand sides
+    print(f"case square xy self N={N}", flush=True)
     mem, fur = _mem_self(gila, orc, N, np.complex64)
     x = orc.complex_gaussian(N + (3, 2), seed=70)
     y = _dev_apply(gila, mem, x, nrhs=2)
@@ -285,11 +284,10 @@ def test_fused_xy_self(gila, orc, gpu, N):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("S,T", FUSED_EXT)
-def test_fused_xy_external_spill_and_zero_rows(gila, orc, gpu, S, T):
-    # S2 > B2 (spill-over rows added) and S2 < B2 (zero rows) at fused-kernel
-    # shapes (B1 = B2)
-    print(f"case fused xy ext S={S} T={T}", flush=True)
+@pytest.mark.parametrize("S,T", SQUARE_EXT)
+def test_square_xy_external_spill_and_zero_rows(gila, orc, gpu, S, T):
+    # S2 > B2 (spill-over rows added) and S2 < B2 (zero rows) with B1 = B2
+    print(f"case square xy ext S={S} T={T}", flush=True)
     delta = (0.25, -0.5, 0.75)
     crc = orc.crc_ext(S, T, (H, H, H), delta)
     sym = (0, 0, 0)
