@@ -15,6 +15,11 @@ Prints one JSON line.  Fields beyond the driver's contract:
   cpu_baseline    the CPU oracle (a C restatement of the reference's host
                   algorithm) timed on this host on a bounded sample
   parity          relative L2 of both GPU forms against the FP64 oracle
+  external        the external operator with uneven volumes (256x34x34
+                  source inside a 256x256x128 target), both precisions
+  batched         64 right-hand sides on a 96^3 operator in one launch, and
+                  a mixed batch of 16 operators replayed from a HIP graph
+  serving         the served solve stream against sequential FP64 solves
 """
 import argparse
 import ctypes
@@ -43,6 +48,7 @@ def parse():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-external", action="store_true", help="skip the external-operator leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched right-hand-side leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served solve-stream leg")
     ap.add_argument("--serve-requests", type=int, default=40)
@@ -241,6 +247,44 @@ def batched_leg(gila, log):
     return out
 
 
+def external_leg(gila, log, steps=20, warmup=3):
+    """External operator with uneven volumes: a 256x34x34 source centred in a
+    256x256x128 target at 1/32 wavelength (B = (256,145,81); axes 2 and 3 run
+    on the next compiled lengths 160 and 96).  Input: an x-polarised point
+    dipole in a seeded random source cell plus 1e-3 complex Gaussian noise.
+    ComplexF32 and ComplexF64, Green data from the library's FP64 setup; the
+    same two timed regions as the headline matvec."""
+    import torch
+    from fractions import Fraction
+    from gila_amd import inputs
+    S, T = (256, 34, 34), (256, 256, 128)
+    h = (Fraction(1, 32),) * 3
+    src, tgt = gila.GlaVol(S, h), gila.GlaVol(T, h)
+    u = inputs.uniform(77, 3)
+    cell = [int(u[d] * S[d]) for d in range(3)]
+    x = 1e-3 * inputs.complex_gaussian(S + (3,), seed=78)
+    x[tuple(cell) + (0,)] += 1.0
+    out = {"workload": "external operator, source 256x34x34 centred in a 256x256x128 target, cells of side 1/32 "
+                       "wavelength, one matvec per step (x-polarised point dipole plus 1e-3 noise)"}
+    for typ, key in ((np.complex64, "c64"), (np.complex128, "c128")):
+        log(f"external: setup {key}")
+        t0 = time.perf_counter()
+        mem = gila.GlaOprMem(gila.GlaKerOpt(True), tgt, src, setTyp=typ)
+        setup_s = time.perf_counter() - t0
+        xd = torch.from_numpy(np.asfortranarray(x.astype(typ)).ravel(order="F").copy()).cuda()
+        yd = torch.empty(3 * T[0] * T[1] * T[2], dtype=xd.dtype, device=xd.device)
+        ms, _, prof, _ = time_matvec(gila, mem, xd, yd, steps, warmup)
+        by = sum(q["bytes"] for q in prof)
+        out[key] = {"ms_per_matvec": ms, "matvecs_per_s": 1000.0 / ms, "bytes_alg_per_matvec": by,
+                    "hbm_gbs_whole_matvec": by / (ms * 1e6), "branch_lengths": list(mem._dims.brn),
+                    "device_lengths": list(mem._dims.dev_brn), "green_setup_s": setup_s, "passes": prof}
+        log(f"external {key}: {ms:.4f} ms/matvec")
+        del mem, xd, yd
+        torch.cuda.empty_cache()
+    out["fp32_over_fp64"] = out["c128"]["ms_per_matvec"] / out["c64"]["ms_per_matvec"]
+    return out
+
+
 def serve_leg(gila, a, log):
     """Served solve stream against sequential plain FP64 solves of the same
     requests (GMRES(30) on (I - chi G) E = E_inc to 1e-6 by the FP64
@@ -415,6 +459,8 @@ def main():
                          "c128_rel_l2_vs_fp64_oracle": orc.rel_l2(r64["y"], y_ref)}
     del res, r32, r64
     torch.cuda.empty_cache()
+    if not a.no_external:
+        out["external"] = external_leg(gila, log)
     if not a.no_batched:
         out["batched"] = batched_leg(gila, log)
     if not a.no_serve:
