@@ -44,7 +44,10 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # about 70 ms of untimed matvecs: after a handful the chip has not reached
+    # its sustained state yet (128^3 FP32: 365 us per matvec after 5, 346 us
+    # after 200)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-reps", type=int, default=3)
@@ -247,7 +250,7 @@ def batched_leg(gila, log):
     return out
 
 
-def external_leg(gila, log, steps=20, warmup=3):
+def external_leg(gila, log, steps=20, warmup=50):
     """External operator with uneven volumes: a 256x34x34 source centred in a
     256x256x128 target at 1/32 wavelength (B = (256,145,81); axes 2 and 3 run
     on the next compiled lengths 160 and 96).  Input: an x-polarised point
