@@ -22,6 +22,19 @@
  *     the reference's face-pair surface-integral formulation evaluated with
  *     fixed-order Gauss-Legendre rules; see the function header).
  *
+ * How each part is pinned (tests/test_oracle.py): the FFT against the DFT
+ * definition and numpy; the matvec restatement against gor_brute_crc on self,
+ * odd-size and external grids, and gor_brute_crc against a direct
+ * block-Toeplitz sum built from the real-space tensor with the reflection
+ * signs; the Fourier storage by the mirror identity of the stored branches;
+ * linearity, reciprocity and the self-vs-external translation identity of
+ * test/extSlfTest.jl:45-47; the face-pair tensor's far field against the
+ * analytic dyadic of test/anaTest.jl:36-44 with the cube-averaging error
+ * (kh)^2/12.  Parity unpinned: the face-pair values for touching and
+ * near cells against the reference's own (adaptive hcubature and DIRECTFN
+ * rules, src/glaGen.jl:350-598, not runnable here); the matvec parity does not
+ * depend on them, since it runs on the same Green data on both sides.
+ *
  * Build: oracle/Makefile -> oracle/_build/libgila_oracle.so
  */
 #include <complex.h>
