@@ -420,24 +420,54 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
     const int m2 = tile >> 16;
     const bool act = m1 < L1;
     const int64_t cs = (int64_t)Z * L2 * L1;      // component stride
-    cpx<R>* col = W2 + (int64_t)rhs * w2s + (int64_t)m2 * L1 + m1;
     const int64_t zs = (int64_t)L2 * L1;
+    // Column addressing: a workgroup-uniform base (scalar registers) plus one
+    // 32-bit per-lane byte offset (xl + t*zs elements; the launcher checks
+    // that TP*L1*L2 elements fit in 32-bit byte offsets), so no access needs
+    // a 64-bit multiply.  cp(c, k) is element (c, k + t) of this thread's
+    // column m1, k uniform.
+    cpx<R>* const wgc = W2 + (int64_t)rhs * w2s + (int64_t)m2 * L1 + (int64_t)(tile & 0xffff) * PX;
+    const uint32_t lob = ((uint32_t)xl + (uint32_t)t * (uint32_t)zs) * (uint32_t)sizeof(cpx<R>);
+    auto cp = [&](int c, int k) -> cpx<R>* {
+        return reinterpret_cast<cpx<R>*>(reinterpret_cast<char*>(wgc + ((int64_t)c * cs + (int64_t)k * zs)) + lob);
+    };
 
     // RELOAD: read the column again for the odd parity (an L2 hit) instead of
     // keeping it in registers through the even parity's transforms
     constexpr int XE = RELOAD ? 1 : E;
+    // full: every lane of the workgroup reads rows 0..N-1 of an existing
+    // column (whole tile inside L1, S3 >= N), so the first-half loads need no
+    // guards and issue back to back; guarded loads otherwise
+    const bool full = (tile & 0xffff) * PX + PX <= L1 && S3 >= N;
+    auto load_first_half = [&](cpx<R>(&w)[3][E]) {
+        if (full) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int e = 0; e < E; ++e) w[c][e] = *cp(c, TP * e);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int n = t + TP * e;
+                    w[c][e] = (act && n < S3) ? *cp(c, TP * e) : cpx<R>{R(0), R(0)};
+                }
+        }
+    };
     cpx<R> xa[3][XE];
     cpx<R> xb[MODE == 1 ? 3 : 1][MODE == 1 ? XE : 1];
     if constexpr (RELOAD == 0) {
+        load_first_half(xa);
+        if constexpr (MODE == 1) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+            for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int n = t + TP * e;
-                xa[c][e] = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
-                if constexpr (MODE == 1)
-                    xb[c][e] = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
-            }
+                for (int e = 0; e < E; ++e) {
+                    const int n = t + TP * e;
+                    xb[c][e] = (act && n + N < S3) ? *cp(c, TP * e + N) : cpx<R>{R(0), R(0)};
+                }
+        }
     }
 
     // per-column branch parities and stored indices on axes 1 and 2
@@ -500,15 +530,16 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
         // both parities split first: the input dies at the split, so the
         // live state is the two parities' fields only
         cpx<R> wv[2][3][E];
+        load_first_half(wv[0]);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int n = t + TP * e;
-                const cpx<R> a = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
+                const cpx<R> a = wv[0][c][e];
                 cpx<R> b = {R(0), R(0)};
                 if constexpr (MODE == 1)
-                    b = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
+                    b = (act && n + N < S3) ? *cp(c, TP * e + N) : cpx<R>{R(0), R(0)};
                 wv[0][c][e] = a + b;
                 wv[1][c][e] = phu[n] * (a - b);
             }
@@ -528,9 +559,9 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
             for (int e = 0; e < E; ++e) {
                 const int n = t + TP * e;
                 const cpx<R> po = cmulc(wv[1][c][e], phu[n]);
-                if (n < top) col[c * cs + (int64_t)n * zs] = wv[0][c][e] + po;
+                if (n < top) *cp(c, TP * e) = wv[0][c][e] + po;
                 if constexpr (MODE == 2)
-                    if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = wv[0][c][e] - po;
+                    if (n + N < T3) *cp(c, TP * e + N) = wv[0][c][e] - po;
             }
         return;
     }
@@ -548,9 +579,9 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
                 const int n = t + TP * e;
                 cpx<R> a, b = {R(0), R(0)};
                 if constexpr (RELOAD) {
-                    a = (act && n < S3) ? col[c * cs + (int64_t)n * zs] : cpx<R>{R(0), R(0)};
+                    a = (act && n < S3) ? *cp(c, TP * e) : cpx<R>{R(0), R(0)};
                     if constexpr (MODE == 1)
-                        b = (act && n + N < S3) ? col[c * cs + (int64_t)(n + N) * zs] : cpx<R>{R(0), R(0)};
+                        b = (act && n + N < S3) ? *cp(c, TP * e + N) : cpx<R>{R(0), R(0)};
                 } else {
                     a = xa[c][e];
                     if constexpr (MODE == 1) b = xb[c][e];
@@ -583,9 +614,9 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_zpass(cpx<R>* __restrict__ W2
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int n = t + TP * e;
-            if (n < top) col[c * cs + (int64_t)n * zs] = acc[c][e];
+            if (n < top) *cp(c, TP * e) = acc[c][e];
             if constexpr (MODE == 2)
-                if (n + N < T3) col[c * cs + (int64_t)(n + N) * zs] = accb[c][e];
+                if (n + N < T3) *cp(c, TP * e + N) = accb[c][e];
         }
 }
 

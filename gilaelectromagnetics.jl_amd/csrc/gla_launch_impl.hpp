@@ -186,7 +186,10 @@ static hipError_t launch_zpass_chunk(const Geom& g, const DevTab& tb, void* W2, 
     dim3 grid(8 * tb.zper_xcd * nrhs, 1, 1);   // right-hand side inside the XCD stretch
     dim3 blk(K::TP_Z * PX);
     const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);
-#define GLA_ZARGS                                                                                   \
+    // k_zpass addresses a thread's column with a 32-bit byte offset
+    // ((column + t * L1 * L2) elements)
+    if ((int64_t)K::TP_Z * g.L[0] * g.L[1] * (int64_t)sizeof(cpx<R>) >= ((int64_t)1 << 32)) return hipErrorInvalidValue;
+#define GLA_ZARGS                                                                                 \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
         tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd, nrhs
     if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
