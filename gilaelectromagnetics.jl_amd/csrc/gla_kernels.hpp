@@ -282,15 +282,32 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restric
     const int m1 = blockIdx.x * PX + xl;
     const bool act = m1 < L1;
     const int i3 = blockIdx.y % S3, c = blockIdx.y / S3;
-    const cpx<R>* src = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + m1;
-    cpx<R>* dst = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + m1;
+    // row k + t of this thread's column: workgroup-uniform base plus a 32-bit
+    // per-lane byte offset (no 64-bit multiply per access)
+    const uint32_t lob = ((uint32_t)xl + (uint32_t)t * (uint32_t)L1) * (uint32_t)sizeof(cpx<R>);
+    const cpx<R>* const su = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + (int64_t)blockIdx.x * PX;
+    cpx<R>* const du = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 +
+                       (int64_t)blockIdx.x * PX;
+    auto sp = [&](int k) { return reinterpret_cast<const cpx<R>*>(reinterpret_cast<const char*>(su + (int64_t)k * L1) + lob); };
+    auto dp = [&](int k) { return reinterpret_cast<cpx<R>*>(reinterpret_cast<char*>(du + (int64_t)k * L1) + lob); };
     cpx<R> v[2][E];
+    // whole bundle inside L1 and S2 >= N: the first-half loads need no guards
+    if ((int)blockIdx.x * PX + PX <= L1 && S2 >= N) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[0][e] = *sp(TP * e);
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int n = t + TP * e;
+            v[0][e] = (act && n < S2) ? *sp(TP * e) : cpx<R>{R(0), R(0)};
+        }
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
-        cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-        if (act && n < S2) a = src[(int64_t)n * L1];
-        if (act && n + N < S2) b = src[(int64_t)(n + N) * L1];
+        const cpx<R> a = v[0][e];
+        cpx<R> b = {R(0), R(0)};
+        if (act && n + N < S2) b = *sp(TP * e + N);
         v[0][e] = a + b;
         v[1][e] = phz[n] * (a - b);
     }
@@ -303,8 +320,8 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restric
     if (act) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            dst[(int64_t)(t + TP * e) * L1] = v[0][e];
-            dst[(int64_t)(N + t + TP * e) * L1] = v[1][e];
+            *dp(TP * e) = v[0][e];
+            *dp(N + TP * e) = v[1][e];
         }
     }
 }
@@ -330,14 +347,28 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
     const int m1 = blockIdx.x * PX + xl;
     const bool act = m1 < L1;
     const int i3 = blockIdx.y % T3, c = blockIdx.y / T3;
-    const cpx<R>* src = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 + m1;
-    cpx<R>* dst = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + m1;
+    // row k + t of this thread's column: workgroup-uniform base plus a 32-bit
+    // per-lane byte offset (no 64-bit multiply per access)
+    const uint32_t lob = ((uint32_t)xl + (uint32_t)t * (uint32_t)L1) * (uint32_t)sizeof(cpx<R>);
+    const cpx<R>* const su = W2 + (int64_t)blockIdx.z * w2s + ((int64_t)c * Z + i3) * (int64_t)(2 * N) * L1 +
+                             (int64_t)blockIdx.x * PX;
+    cpx<R>* const du = W1 + (int64_t)blockIdx.z * w1s + ((int64_t)c * Z + i3) * Y1 * L1 + (int64_t)blockIdx.x * PX;
+    auto sp = [&](int k) { return reinterpret_cast<const cpx<R>*>(reinterpret_cast<const char*>(su + (int64_t)k * L1) + lob); };
+    auto dp = [&](int k) { return reinterpret_cast<cpx<R>*>(reinterpret_cast<char*>(du + (int64_t)k * L1) + lob); };
     cpx<R> v[2][E];
+    // whole bundle inside L1: the loads need no guards
+    if ((int)blockIdx.x * PX + PX <= L1) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int k = t + TP * e;
-        v[0][e] = act ? src[(int64_t)k * L1] : cpx<R>{R(0), R(0)};
-        v[1][e] = act ? src[(int64_t)(N + k) * L1] : cpx<R>{R(0), R(0)};
+        for (int e = 0; e < E; ++e) {
+            v[0][e] = *sp(TP * e);
+            v[1][e] = *sp(N + TP * e);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            v[0][e] = act ? *sp(TP * e) : cpx<R>{R(0), R(0)};
+            v[1][e] = act ? *sp(N + TP * e) : cpx<R>{R(0), R(0)};
+        }
     }
     if constexpr (YK<R>::value == 2) {
         PencilFft<R, N, TP, PX, 1, true, 0, 2, PX * N>::run(v, lds, t, xl, tw);
@@ -351,8 +382,8 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
         const cpx<R> po = cmulc(v[1][e], phz[n]);
-        if (n < top) dst[(int64_t)n * L1] = v[0][e] + po;
-        if (n + N < T2) dst[(int64_t)(n + N) * L1] = v[0][e] - po;
+        if (n < top) *dp(TP * e) = v[0][e] + po;
+        if (n + N < T2) *dp(TP * e + N) = v[0][e] - po;
     }
 }
 
