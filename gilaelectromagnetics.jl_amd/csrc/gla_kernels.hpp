@@ -150,6 +150,13 @@ __device__ __forceinline__ void g_multiply(cpx<R> (&w)[3][E], const cpx<R>* __re
     }
 }
 
+// GLA_XK: the two parities of an axis-1 pencil go through the transform
+// stages together (2, two LDS buffers) or one after the other through one
+// buffer (1: half the LDS per workgroup, so more workgroups fit a CU)
+#ifndef GLA_XK
+#define GLA_XK 1
+#endif
+
 // --------------------------------------------------------------------------
 // Pass 1: x-forward on contiguous rows.  One pencil = one (i2,i3,c) row of X.
 // --------------------------------------------------------------------------
@@ -166,7 +173,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
     constexpr int PS = N + (N >> PADSH) + 1;
     // rows are whole pencils inside one wave when TP divides 64
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
-    __shared__ cpx<R> lds[2 * PEN * PS];
+    __shared__ cpx<R> lds[GLA_XK * PEN * PS];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PEN);
@@ -190,7 +197,13 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
         v[0][e] = a + b;
         v[1][e] = phz[n] * (a - b);
     }
-    PencilFft<R, N, TP, 1, PS, false, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
+    if constexpr (GLA_XK == 2) {
+        PencilFft<R, N, TP, 1, PS, false, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
+    } else {
+        PencilFft<R, N, TP, 1, PS, false, XSYNC, 1, 0, PADSH>::run(v[0], lds, t, pen, tw);
+        lds_sync<XSYNC>();
+        PencilFft<R, N, TP, 1, PS, false, XSYNC, 1, 0, PADSH>::run(v[1], lds, t, pen, tw);
+    }
     if (act) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -219,7 +232,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     constexpr int PADSH = 3;
     constexpr int PS = N + (N >> PADSH) + 1;
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
-    __shared__ cpx<R> lds[2 * PEN * PS];
+    __shared__ cpx<R> lds[GLA_XK * PEN * PS];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PEN);
@@ -241,7 +254,13 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
         v[0][e] = act ? wr[k] : cpx<R>{R(0), R(0)};
         v[1][e] = act ? wr[N + k] : cpx<R>{R(0), R(0)};
     }
-    PencilFft<R, N, TP, 1, PS, true, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
+    if constexpr (GLA_XK == 2) {
+        PencilFft<R, N, TP, 1, PS, true, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
+    } else {
+        PencilFft<R, N, TP, 1, PS, true, XSYNC, 1, 0, PADSH>::run(v[0], lds, t, pen, tw);
+        lds_sync<XSYNC>();
+        PencilFft<R, N, TP, 1, PS, true, XSYNC, 1, 0, PADSH>::run(v[1], lds, t, pen, tw);
+    }
     if (!act) return;
     const int top = T1 < N ? T1 : N;
     const bool rd = axpby == 1 && !(beta.x == R(0) && beta.y == R(0));
