@@ -4,16 +4,16 @@
 #include "gla_launch.hpp"
 
 // kernel-shape defaults (overridable at compile time for variant timing)
-#ifndef GLA_EROW
-#define GLA_EROW 8
+// Rows of passes 1, 2, 4, 5: target elements per thread by length (same-GPU
+// timing against 8 everywhere, after the row-pass addressing change: 4 is
+// faster up to 128 in both types -- 128^3 ComplexF32 322.5 -> 317.5 us,
+// ComplexF64 651.8 -> 638.7 us per matvec -- and at 160 in ComplexF32, slower
+// at 256 and at 160 in ComplexF64).  GLA_EROW_F / GLA_EROW_D force one value.
+#ifndef GLA_EROW_SMALL_F
+#define GLA_EROW_SMALL_F 160
 #endif
-// ComplexF32 rows: 4 elements per thread (same-GPU timing at 128^3: 322.5 ->
-// 317.5 us per matvec against 8, after the row-pass addressing change)
-#ifndef GLA_EROW_F
-#define GLA_EROW_F 4
-#endif
-#ifndef GLA_EROW_D
-#define GLA_EROW_D GLA_EROW
+#ifndef GLA_EROW_SMALL_D
+#define GLA_EROW_SMALL_D 128
 #endif
 // pass-3 schedule: 0 keeps the input column in registers through the even
 // parity, 2 splits both parities first (fewer registers for FP64)
@@ -98,7 +98,15 @@ template <typename R>
 struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
     static constexpr int PX = DBL ? GLA_PXY_D : GLA_PXY_F;   // axis-2 passes: columns per bundle
-    static constexpr int E_ROW = DBL ? GLA_EROW_D : GLA_EROW_F;   // passes 1, 2, 4, 5
+    // passes 1, 2, 4, 5
+    static constexpr int e_row(int n)
+    {
+#if defined(GLA_EROW_F) && defined(GLA_EROW_D)
+        return DBL ? GLA_EROW_D : GLA_EROW_F;
+#else
+        return n <= (DBL ? GLA_EROW_SMALL_D : GLA_EROW_SMALL_F) ? 4 : 8;
+#endif
+    }
     static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
     static constexpr int ZMINW = DBL ? GLA_ZMINW_D : GLA_ZMINW_F;
     static constexpr int ZGG = DBL ? GLA_ZGG_D : GLA_ZGG_F;
@@ -108,7 +116,7 @@ struct Cfg {
 template <typename R, int N>
 struct LenK {
     using C = Cfg<R>;
-    static constexpr int TP_ROW = N / pick_e(N, C::E_ROW, 64, true);
+    static constexpr int TP_ROW = N / pick_e(N, C::e_row(N), 64, true);
     static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
     // axis-2 passes: the bundle width, capped so the two parities' LDS
     // buffers and the staged tables stay within GLA_YLDS_KB
@@ -117,7 +125,7 @@ struct LenK {
         while (px > 1 && (YK<R>::value * px + (GLA_XYTAB ? 2 : 0)) * N * (int)sizeof(cpx<R>) > GLA_YLDS_KB * 1024) px /= 2;
         return px;
     }();
-    static constexpr int TP_Y = N / pick_e(N, C::E_ROW, 1024 / PXY);
+    static constexpr int TP_Y = N / pick_e(N, C::e_row(N), 1024 / PXY);
     // pass 3: shape from the tuning table (gla_tune.hpp)
     static constexpr int PXZ = ztune(sizeof(cpx<R>), N).px;
     static constexpr int TP_Z = N / pick_e(N, ztune(sizeof(cpx<R>), N).e, 1024 / PXZ, GLA_ZWAVE != 0);
