@@ -76,6 +76,36 @@ bool vol_eq(const gla_vol& a, const gla_vol& b)
     return true;
 }
 
+// Work planes of the allocation-free matvec (gla_opr_apply_noalloc): one copy
+// of this array per device, held from the first gla_init on that device until
+// the process ends.  Its host-side name is not a device address: host code
+// uses the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchBytes = (size_t)GLA_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on `device`, looked up once per device with
+// that device current
+gla_status scratch_addr(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return fail(GLA_ERR_NO_DEVICE, "HIP device " + std::to_string(device) + " is beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        DevGuard g(device);
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GLA_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return GLA_OK;
+}
+
 gla_status check_vol(const gla_vol& v, const char* what)
 {
     for (int d = 0; d < 3; ++d) {
@@ -128,7 +158,8 @@ gla_status gla_init(int32_t device)
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(GLA_ERR_NO_DEVICE, "no HIP device is available");
     if (device < 0 || device >= n)
         return fail(GLA_ERR_NO_DEVICE, "HIP device " + std::to_string(device) + " does not exist");
-    return GLA_OK;
+    void* p = nullptr;
+    return scratch_addr(device, &p);
 }
 
 gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* out)
@@ -619,6 +650,42 @@ gla_status gla_opr_destroy(gla_opr_t h)
 gla_status gla_opr_apply(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs, void* stream)
 {
     return run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);
+}
+
+gla_status gla_opr_apply_noalloc(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
+                                 void* stream)
+{
+    if (!h) return fail(GLA_ERR_ARG, "null operator");
+    if (nrhs < 1 || nrhs > 65535) return fail(GLA_ERR_ARG, "nrhs must be in [1, 65535]");
+    if (!x || !y) return fail(GLA_ERR_ARG, "null vector");
+    if (nx != h->geom.xs * nrhs)
+        return fail(GLA_ERR_SIZE, "Size of input vector does not match size of source volume. Required data "
+                                  "layout is (srcCelX, srcCelY, srcCelZ, 3).");
+    if (ny != h->geom.ys * nrhs) return fail(GLA_ERR_SIZE, "Size of output vector does not match the target volume.");
+    const size_t eb = elem_bytes(h->dims.dtype);
+    // W1 then W2, each start on a 256-byte boundary
+    const size_t w1b = ((size_t)(h->geom.w1s * nrhs) * eb + 255) & ~(size_t)255;
+    const size_t need = w1b + (size_t)(h->geom.w2s * nrhs) * eb;
+    if (need > kScratchBytes)
+        return fail(GLA_ERR_SIZE, "the allocation-free matvec needs " + std::to_string(need) +
+                                      " bytes of work memory, more than its fixed " + std::to_string(kScratchBytes) +
+                                      "-byte scratch (use gla_opr_apply)");
+    void* base = nullptr;
+    gla_status s = scratch_addr(h->opts.device, &base);
+    if (s) return s;
+    std::lock_guard<std::mutex> lk(h->mu);
+    DevGuard g(h->opts.device);
+    void* w1 = base;
+    void* w2 = (unsigned char*)base + w1b;
+    hipError_t e;
+    if (h->dims.dtype == GLA_C64)
+        e = gla::run_matvec<float>(h->geom, h->tab, x, y, w1, w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(),
+                                   (hipStream_t)stream);
+    else
+        e = gla::run_matvec<double>(h->geom, h->tab, x, y, w1, w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(),
+                                    (hipStream_t)stream);
+    if (e != hipSuccess) return fail(GLA_ERR_HIP, std::string("matvec launch: ") + hipGetErrorString(e));
+    return GLA_OK;
 }
 
 gla_status gla_opr_apply_axpby(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,

@@ -143,6 +143,25 @@ gla_status gla_opr_destroy(gla_opr_t h);
 gla_status gla_opr_apply(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
                          void* stream);
 
+/* Size of the fixed per-device scratch of gla_opr_apply_noalloc: 256 MiB
+ * (for instance one right-hand side of a 96^3 ComplexF32 or a 64^3
+ * ComplexF64 self operator). */
+#define GLA_NOALLOC_SCRATCH_BYTES (256LL << 20)
+
+/* The matvec of gla_opr_apply without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * two work planes of the launch sequence live in one fixed array of
+ * GLA_NOALLOC_SCRATCH_BYTES per device (held from the first gla_init on that
+ * device until the process ends) instead of the handle's workspace; a call
+ * whose planes do not fit fails with GLA_ERR_SIZE before launching anything.
+ * Arguments, layout and results are those of gla_opr_apply.  Every call on
+ * one GPU shares that array: calls that overlap on the GPU (different
+ * streams, or different handles on one device) race and must be serialized
+ * by the caller. */
+gla_status gla_opr_apply_noalloc(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
+                                 void* stream);
+
 /* mul!(y, G, x, alpha, beta) (src/glaLinAlg.jl:61-69): y = alpha*G*x + beta*y,
  * fused into the last pass.  beta == 0 never reads y. */
 gla_status gla_opr_apply_axpby(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int64_t nrhs,
