@@ -3,8 +3,14 @@
 //      (the reference's formulation: egoFunInn! / egoSrfAdp! / srfKer /
 //      srfSum!, src/glaGen.jl:322-338, 531-598; cube faces and scalings of
 //      cubFac / srfScl, src/glaInt.jl:284-328; scalar kernel sclEgo,
-//      src/glaInt.jl:41-44), with fixed tensor Gauss-Legendre rules whose
-//      order falls with the separation instead of adaptive cubature;
+//      src/glaInt.jl:41-44).  Each face-pair integral is reduced to difference
+//      coordinates (2-D for parallel faces, 3-D for perpendicular ones, with
+//      the faces' overlap lengths as weights) and integrated with tensor
+//      Gauss-Legendre rules on the pieces between the weights' kinks, to about
+//      1e-14 relative -- in place of the reference's adaptive hcubature at
+//      1e-6; for cells that touch on the cell grid the face pairs that
+//      egoFunSng! marks (src/glaGen.jl:350-525) take the DIRECTFN singular
+//      values of gla_singular.hip, as the reference's do;
 //   2. the circulant embedding (egoToeCrc!, src/glaGen.jl:143-163, for self
 //      operators; the separation grid of egoFunExt! / grdSel,
 //      src/glaGen.jl:169-189, 632-643, for external ones);
@@ -25,114 +31,343 @@
 namespace gla {
 namespace {
 
-constexpr int QMAX = 16;   // largest Gauss-Legendre order in the table
+constexpr int QMAX = 24;   // largest Gauss-Legendre order in the table
 
 struct GreenCfg {
     double scl[3];          // cell size per axis (wavelengths)
     double zr, zi;          // frqPhz
-    int qnear;              // order for touching cells (source; target uses qnear + 1)
+    double kr, ki;          // 2 pi frqPhz
+    double cr, ci;          // 1 / (4 pi frqPhz^2)
+    int qphase;             // smallest regular order for the cell's phase span
 };
 
-__device__ __forceinline__ void cexp_i(double re, double im, double& o_re, double& o_im)
+struct Cplx {
+    double r, i;
+};
+
+// r g(r) = exp(2 pi i zeta r) / (4 pi zeta^2), smooth at r = 0
+__device__ __forceinline__ Cplx rg(const GreenCfg& G, double r)
 {
-    // exp(i (re + i im)) = exp(-im) (cos re + i sin re)
     double s, c;
-    sincos(re, &s, &c);
-    const double m = exp(-im);
-    o_re = m * c;
-    o_im = m * s;
+    sincos(G.kr * r, &s, &c);
+    const double m = G.ki == 0.0 ? 1.0 : exp(-G.ki * r);
+    return Cplx{m * (c * G.cr - s * G.ci), m * (c * G.ci + s * G.cr)};
 }
 
-// integral over the unit square pair of g(|r_t - r_s + sep|), faces given by
-// normal axis (nt, ns) and side (st, ss: 0 lower, 1 upper)
-__device__ void face_pair(const GreenCfg& G, const double sep[3], int nt, int st, int ns, int ss, int qs, int qt,
-                          const double* __restrict__ gl, double& acc_re, double& acc_im)
+__device__ __forceinline__ const double* gl_tab(const double* gl, int q) { return gl + (q * (q - 1) / 2) * 2; }
+
+// overlap length at difference w of two intervals of length h whose centres
+// differ by c: the tent h - |w - c| on [c - h, c + h]
+__device__ __forceinline__ double tent(double w, double c, double h) { return fmax(0.0, h - fabs(w - c)); }
+
+constexpr double SNAP = 1e-12;   // relative: a breakpoint this close to zero is zero
+
+// sorted, distinct breakpoints of [lo, hi]: the ends, the interior kink
+// `mid` (NaN: none) and, when `zero` and it lies inside, the origin; values
+// within SNAP * h of zero are zero.  Returns their count (at most 4).
+__device__ int breaks(double lo, double mid, double hi, bool zero, double h, double* b)
 {
-    // node/weight tables: gl[(q*(q-1)/2 + i)*2 + {0,1}] for order q, node i
-    const double* xs = gl + (qs * (qs - 1) / 2) * 2;
-    const double* xt = gl + (qt * (qt - 1) / 2) * 2;
-    // source face: coordinate ns fixed, the other two spanned
-    int sa = (ns + 1) % 3, sb = (ns + 2) % 3;
-    int ta = (nt + 1) % 3, tb = (nt + 2) % 3;
-    const double h[3] = {G.scl[0], G.scl[1], G.scl[2]};
-    const double zr = G.zr, zi = G.zi;
-    // 2 pi zeta and 4 pi zeta^2
-    const double kr = 2.0 * M_PI * zr, ki = 2.0 * M_PI * zi;
-    const double z2r = zr * zr - zi * zi, z2i = 2.0 * zr * zi;
-    double ar = 0, ai = 0;
-    for (int i1 = 0; i1 < qs; ++i1)
-        for (int i2 = 0; i2 < qs; ++i2) {
-            double ps[3];
-            ps[ns] = (ss ? 0.5 : -0.5) * h[ns];
-            ps[sa] = (xs[2 * i1] - 0.5) * h[sa];
-            ps[sb] = (xs[2 * i2] - 0.5) * h[sb];
-            const double ws = xs[2 * i1 + 1] * xs[2 * i2 + 1];
-            for (int j1 = 0; j1 < qt; ++j1)
-                for (int j2 = 0; j2 < qt; ++j2) {
-                    double pt[3];
-                    pt[nt] = (st ? 0.5 : -0.5) * h[nt];
-                    pt[ta] = (xt[2 * j1] - 0.5) * h[ta];
-                    pt[tb] = (xt[2 * j2] - 0.5) * h[tb];
-                    const double dx = pt[0] - ps[0] + sep[0], dy = pt[1] - ps[1] + sep[1], dz = pt[2] - ps[2] + sep[2];
-                    const double d = sqrt(dx * dx + dy * dy + dz * dz);
-                    // g = exp(i 2 pi d zeta) / (4 pi d zeta^2)
-                    double er, ei;
-                    cexp_i(kr * d, ki * d, er, ei);
-                    const double dr = 4.0 * M_PI * d * z2r, di = 4.0 * M_PI * d * z2i;
-                    const double inv = 1.0 / (dr * dr + di * di);
-                    const double gr = (er * dr + ei * di) * inv, gi = (ei * dr - er * di) * inv;
-                    const double w = ws * xt[2 * j1 + 1] * xt[2 * j2 + 1];
-                    ar += w * gr;
-                    ai += w * gi;
-                }
+    double c[4];
+    int n = 0;
+    c[n++] = lo;
+    if (!isnan(mid)) c[n++] = mid;
+    c[n++] = hi;
+    if (zero && lo < 0.0 && hi > 0.0) c[n++] = 0.0;
+    for (int k = 0; k < n; ++k)
+        if (fabs(c[k]) < SNAP * h) c[k] = 0.0;
+    for (int k = 1; k < n; ++k)
+        for (int j = k; j > 0 && c[j] < c[j - 1]; --j) {
+            const double t = c[j];
+            c[j] = c[j - 1];
+            c[j - 1] = t;
         }
-    acc_re = ar;
-    acc_im = ai;
+    int m = 0;
+    for (int k = 0; k < n; ++k)
+        if (m == 0 || c[k] > b[m - 1]) b[m++] = c[k];
+    return m;
 }
 
-// six upper-triangle components at separation sep; srfSum! combination
-__device__ void green_at(const GreenCfg& G, const double sep[3], const double* __restrict__ gl, double out[12])
+// Face pair of parallel faces, in difference coordinates: the integral over
+// (wa, wb) of g(sqrt(wa^2 + wb^2 + d^2)) tent_a(wa) tent_b(wb), tents centred
+// at ca, cb with half widths ha, hb.  Pieces between the tents' kinks use a
+// q x q rule; with d = 0 and the origin in the support, the pieces are also
+// split at the origin and those with the origin at a corner are integrated in
+// polar coordinates about it (r g(r) is smooth there).
+__device__ Cplx par_pair(const GreenCfg& G, double d, double ca, double ha, double cb, double hb, int q, int qs,
+                         const double* gl)
+{
+    const double hmax = fmax(ha, hb);
+    if (fabs(d) < SNAP * hmax) d = 0.0;
+    const bool sing = d == 0.0 && ca - ha <= SNAP * ha && ca + ha >= -SNAP * ha && cb - hb <= SNAP * hb &&
+                      cb + hb >= -SNAP * hb;
+    double ba[5], bb[5];
+    const int na = breaks(ca - ha, ca, ca + ha, sing, ha, ba), nb = breaks(cb - hb, cb, cb + hb, sing, hb, bb);
+    const double* x = gl_tab(gl, q);
+    const double* xs = gl_tab(gl, qs);
+    double ar = 0, ai = 0;
+    for (int i = 0; i + 1 < na; ++i)
+        for (int j = 0; j + 1 < nb; ++j) {
+            const double a0 = ba[i], a1 = ba[i + 1], b0 = bb[j], b1 = bb[j + 1];
+            if (sing && (a0 == 0.0 || a1 == 0.0) && (b0 == 0.0 || b1 == 0.0)) {
+                const double su = a0 == 0.0 ? 1.0 : -1.0, sv = b0 == 0.0 ? 1.0 : -1.0;
+                const double A = a1 - a0, B = b1 - b0, t0 = atan2(B, A);
+                for (int tri = 0; tri < 2; ++tri) {
+                    const double ta = tri ? t0 : 0.0, tb = tri ? 0.5 * M_PI : t0;
+                    for (int k = 0; k < qs; ++k) {
+                        const double t = ta + (tb - ta) * xs[2 * k], wt = (tb - ta) * xs[2 * k + 1];
+                        double st, ct;
+                        sincos(t, &st, &ct);
+                        const double rm = tri ? B / st : A / ct;
+                        for (int l = 0; l < qs; ++l) {
+                            const double r = rm * xs[2 * l], w = wt * rm * xs[2 * l + 1];
+                            const Cplx g = rg(G, r);
+                            const double f = w * tent(su * r * ct, ca, ha) * tent(sv * r * st, cb, hb);
+                            ar += f * g.r;
+                            ai += f * g.i;
+                        }
+                    }
+                }
+                continue;
+            }
+            for (int k = 0; k < q; ++k) {
+                const double wa = a0 + (a1 - a0) * x[2 * k], fa = (a1 - a0) * x[2 * k + 1] * tent(wa, ca, ha);
+                for (int l = 0; l < q; ++l) {
+                    const double wb = b0 + (b1 - b0) * x[2 * l];
+                    const double r = sqrt(wa * wa + wb * wb + d * d);
+                    const Cplx g = rg(G, r);
+                    const double f = fa * (b1 - b0) * x[2 * l + 1] * tent(wb, cb, hb) / r;
+                    ar += f * g.r;
+                    ai += f * g.i;
+                }
+            }
+        }
+    return Cplx{ar, ai};
+}
+
+// Face pair of perpendicular faces: the integral of tent_c(w) g(sqrt(w^2 +
+// u^2 + v^2)) over w (tent centred at cc, half width hc, along the shared
+// axis), u in [u0, u1] and v in [v0, v1].  Regular pieces use a q^3 rule; with
+// the origin in the closed box the ranges are also split at the origin and a
+// piece with the origin at a corner is integrated over three Duffy pyramids
+// with their apex there.
+__device__ Cplx perp_pair(const GreenCfg& G, double cc, double hc, double u0, double u1, double v0, double v1, int q,
+                          int qs, const double* gl)
+{
+    const double h = fmax(hc, fmax(u1 - u0, v1 - v0));
+    const bool sing = cc - hc <= SNAP * h && cc + hc >= -SNAP * h && u0 <= SNAP * h && u1 >= -SNAP * h &&
+                      v0 <= SNAP * h && v1 >= -SNAP * h;
+    double bw[5], bu[4], bv[4];
+    const int nw = breaks(cc - hc, cc, cc + hc, sing, h, bw);
+    const int nu = breaks(u0, NAN, u1, sing, h, bu), nv = breaks(v0, NAN, v1, sing, h, bv);
+    const double* x = gl_tab(gl, q);
+    const double* xs = gl_tab(gl, qs);
+    double ar = 0, ai = 0;
+    for (int i = 0; i + 1 < nw; ++i)
+        for (int j = 0; j + 1 < nu; ++j)
+            for (int k = 0; k + 1 < nv; ++k) {
+                const double w0 = bw[i], w1 = bw[i + 1], a0 = bu[j], a1 = bu[j + 1], b0 = bv[k], b1 = bv[k + 1];
+                if (sing && (w0 == 0.0 || w1 == 0.0) && (a0 == 0.0 || a1 == 0.0) && (b0 == 0.0 || b1 == 0.0)) {
+                    const double sw = w0 == 0.0 ? 1.0 : -1.0;
+                    const double L[3] = {w1 - w0, a1 - a0, b1 - b0};
+                    for (int ax = 0; ax < 3; ++ax) {
+                        const int o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;
+                        for (int e = 0; e < qs; ++e) {
+                            const double t = xs[2 * e], wt = xs[2 * e + 1];
+                            for (int f1 = 0; f1 < qs; ++f1) {
+                                const double s1 = L[o1] * xs[2 * f1], w1_ = L[o1] * xs[2 * f1 + 1];
+                                for (int f2 = 0; f2 < qs; ++f2) {
+                                    const double s2 = L[o2] * xs[2 * f2], w2_ = L[o2] * xs[2 * f2 + 1];
+                                    double P[3];
+                                    P[ax] = t * L[ax];
+                                    P[o1] = t * s1;
+                                    P[o2] = t * s2;
+                                    const double rho = sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
+                                    const Cplx g = rg(G, rho);
+                                    const double f = wt * w1_ * w2_ * t * t * L[ax] / rho * tent(sw * P[0], cc, hc);
+                                    ar += f * g.r;
+                                    ai += f * g.i;
+                                }
+                            }
+                        }
+                    }
+                    continue;
+                }
+                for (int e = 0; e < q; ++e) {
+                    const double w = w0 + (w1 - w0) * x[2 * e], fw = (w1 - w0) * x[2 * e + 1] * tent(w, cc, hc);
+                    for (int l = 0; l < q; ++l) {
+                        const double u = a0 + (a1 - a0) * x[2 * l], fu = fw * (a1 - a0) * x[2 * l + 1];
+                        for (int m = 0; m < q; ++m) {
+                            const double v = b0 + (b1 - b0) * x[2 * m];
+                            const double r = sqrt(w * w + u * u + v * v);
+                            const Cplx g = rg(G, r);
+                            const double f = fu * (b1 - b0) * x[2 * m + 1] / r;
+                            ar += f * g.r;
+                            ai += f * g.i;
+                        }
+                    }
+                }
+            }
+    return Cplx{ar, ai};
+}
+
+// integral of g over target face (normal nt, side st: 0 lower, 1 upper) of
+// the cell at separation sep times source face (ns, ss) of the cell at the
+// origin (the reference's egoSrfAdp! integral times the face areas)
+__device__ Cplx face_pair(const GreenCfg& G, const double sep[3], int nt, int st, int ns, int ss, int q, int qs,
+                          const double* gl)
 {
     const double* h = G.scl;
-    bool touch = true;
-    for (int d = 0; d < 3; ++d)
-        if (fabs(sep[d]) > h[d] + 1e-8) touch = false;
-    const double hm = fmax(h[0], fmax(h[1], h[2]));
-    const double dist = sqrt(sep[0] * sep[0] + sep[1] * sep[1] + sep[2] * sep[2]) / hm;
-    int qs, qt;
-    if (touch) { qs = G.qnear; qt = G.qnear + 1; }
-    else if (dist < 3.0) { qs = qt = 6; }
-    else if (dist < 8.0) { qs = qt = 4; }
-    else if (dist < 24.0) { qs = qt = 3; }
-    else { qs = qt = 2; }
-    // A[i][j]: sum over sides of sign * scale * integral, target normal i,
-    // source normal j; sign s_t s_s (lower +1, upper -1), negated off-diagonal
+    if (nt == ns) {
+        const int a = (nt + 1) % 3, b = (nt + 2) % 3;
+        return par_pair(G, sep[nt] + (st - ss) * h[nt], sep[a], h[a], sep[b], h[b], q, qs, gl);
+    }
+    const int c = 3 - nt - ns;
+    return perp_pair(G, sep[c], h[c], sep[nt] + (st - 1) * h[nt], sep[nt] + st * h[nt], sep[ns] - ss * h[ns],
+                     sep[ns] + (1 - ss) * h[ns], q, qs, gl);
+}
+
+// corVal / mask tables of egoFunSng! (src/glaGen.jl:350-525): entry
+// [o1 + 2 o2 + 4 o3][target face][source face] (faces 2*normal + side, the
+// cubFac order yzL yzU xzL xzU xyL xyU) is 0 for a regular face pair, else
+// 1 + the index into [wS(3) wE(9) wV(6)]
+#define S1 1
+#define S2 2
+#define S3 3
+#define E1 4
+#define E2 5
+#define E3 6
+#define E4 7
+#define E5 8
+#define E6 9
+#define E7 10
+#define E8 11
+#define E9 12
+#define V1 13
+#define V2 14
+#define V3 15
+#define V4 16
+#define V5 17
+#define V6 18
+#define __ 0
+__constant__ unsigned char COR[8][6][6] = {
+    {{S1, __, E7, E7, E8, E8}, {__, S1, E7, E7, E8, E8}, {E7, E7, S2, __, E9, E9},
+     {E7, E7, __, S2, E9, E9}, {E8, E8, E9, E9, S3, __}, {E8, E8, E9, E9, __, S3}},
+    {{__, S1, E7, E7, E8, E8}, {__, __, __, __, __, __}, {__, E7, E3, __, V6, V6},
+     {__, E7, __, E3, V6, V6}, {__, E8, V6, V6, E5, __}, {__, E8, V6, V6, __, E5}},
+    {{E1, __, __, E7, V5, V5}, {__, E1, __, E7, V5, V5}, {E7, E7, __, S2, E9, E9},
+     {__, __, __, __, __, __}, {V5, V5, __, E9, E6, __}, {V5, V5, __, E9, __, E6}},
+    {{__, E1, __, E7, V5, V5}, {__, __, __, __, __, __}, {__, E7, __, E3, V6, V6},
+     {__, __, __, __, __, __}, {__, V5, __, V6, V3, __}, {__, V5, __, V6, __, V3}},
+    {{E2, __, V4, V4, __, E8}, {__, E2, V4, V4, __, E8}, {V4, V4, E4, __, __, E9},
+     {V4, V4, __, E4, __, E9}, {E8, E8, E9, E9, __, S3}, {__, __, __, __, __, __}},
+    {{__, E2, V4, V4, __, E8}, {__, __, __, __, __, __}, {__, V4, V2, __, __, V6},
+     {__, V4, __, V2, __, V6}, {__, E8, V6, V6, __, E5}, {__, __, __, __, __, __}},
+    {{V1, __, __, V4, __, V5}, {__, V1, __, V4, __, V5}, {V4, V4, __, E4, __, E9},
+     {__, __, __, __, __, __}, {V5, V5, __, E9, __, E6}, {__, __, __, __, __, __}},
+    {{__, V1, __, V4, __, V5}, {__, __, __, __, __, __}, {__, V4, __, V2, __, V6},
+     {__, __, __, __, __, __}, {__, V5, __, V6, __, V3}, {__, __, __, __, __, __}}};
+#undef S1
+#undef S2
+#undef S3
+#undef E1
+#undef E2
+#undef E3
+#undef E4
+#undef E5
+#undef E6
+#undef E7
+#undef E8
+#undef E9
+#undef V1
+#undef V2
+#undef V3
+#undef V4
+#undef V5
+#undef V6
+#undef __
+
+// order of the regular rules: by the separation in cells (Chebyshev) and at
+// least the order the cell's phase span needs
+__device__ __forceinline__ int reg_order(const GreenCfg& G, double D)
+{
+    const int q = D < 2.5 ? 12 : (D < 4.5 ? 10 : (D < 8.5 ? 8 : (D < 16.5 ? 6 : (D < 48.5 ? 5 : 4))));
+    return q > G.qphase ? q : G.qphase;
+}
+
+// six upper-triangle components at separation sep (the srfSum! combination,
+// src/glaGen.jl:531-555, written per normal pair): A[i][j] sums target
+// normal i, source normal j with side signs (lower +, upper -), negated for
+// i != j, divided by the cell volume; xx = A_yy + A_zz, ..., xy = A_xy, ...
+// Separations of at most one cell per axis on the cell grid take the
+// singular values w18 (already divided by the volume) for the face pairs the
+// tables mark, at the absolute offset, with the flips of negative axes.
+__device__ void green_at(const GreenCfg& G, const double sep[3], const double* __restrict__ gl,
+                         const double2* __restrict__ w18, double out[12])
+{
+    const double* h = G.scl;
+    bool near = true;
+    int o[3];
+    double f[3], D = 0;
+    for (int d = 0; d < 3; ++d) {
+        const double t = sep[d] / h[d];
+        const double r = rint(t);
+        if (fabs(t - r) > 1e-9 || fabs(r) > 1.0) near = false;
+        o[d] = (int)fabs(r);
+        f[d] = r < 0 ? -1.0 : 1.0;
+        D = fmax(D, fabs(t));
+    }
+    const int q = reg_order(G, D);
+    const int qs = G.qphase > 16 ? G.qphase : 16;
+    const double vol = h[0] * h[1] * h[2];
     double A[3][3][2];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) A[i][j][0] = A[i][j][1] = 0;
-    const int pairs[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
-    for (int p = 0; p < 6; ++p) {
-        const int nt = pairs[p][0], ns = pairs[p][1];
-        // srfScl: area of the target-normal face over the cell length along
-        // the source normal
-        const double area = h[(nt + 1) % 3] * h[(nt + 2) % 3];
-        const double scale = area / h[ns];
-        for (int st = 0; st < 2; ++st)
-            for (int ss = 0; ss < 2; ++ss) {
-                double r, im;
-                face_pair(G, sep, nt, st, ns, ss, qs, qt, gl, r, im);
-                const double sg = ((st == ss) ? 1.0 : -1.0) * (nt == ns ? 1.0 : -1.0) * scale;
-                A[nt][ns][0] += sg * r;
-                A[nt][ns][1] += sg * im;
-            }
+    if (near) {
+        const double as[3] = {o[0] * h[0], o[1] * h[1], o[2] * h[2]};
+        const int key = o[0] + 2 * o[1] + 4 * o[2];
+        for (int nt = 0; nt < 3; ++nt)
+            for (int ns = nt; ns < 3; ++ns)
+                for (int st = 0; st < 2; ++st)
+                    for (int ss = 0; ss < 2; ++ss) {
+                        const int code = COR[key][2 * nt + st][2 * ns + ss];
+                        Cplx v;
+                        if (code) {
+                            v = Cplx{w18[code - 1].x, w18[code - 1].y};
+                        } else {
+                            v = face_pair(G, as, nt, st, ns, ss, q, qs, gl);
+                            v.r /= vol;
+                            v.i /= vol;
+                        }
+                        const double sg = (st == ss ? 1.0 : -1.0) * (nt == ns ? 1.0 : -1.0);
+                        A[nt][ns][0] += sg * v.r;
+                        A[nt][ns][1] += sg * v.i;
+                    }
+    } else {
+        for (int n = 0; n < 3; ++n) {
+            // 2 I(d) - I(d + h) - I(d - h) along the normal
+            const Cplx i0 = face_pair(G, sep, n, 0, n, 0, q, qs, gl);
+            const Cplx ip = face_pair(G, sep, n, 1, n, 0, q, qs, gl);
+            const Cplx im = face_pair(G, sep, n, 0, n, 1, q, qs, gl);
+            A[n][n][0] = (2 * i0.r - ip.r - im.r) / vol;
+            A[n][n][1] = (2 * i0.i - ip.i - im.i) / vol;
+        }
+        for (int nt = 0; nt < 3; ++nt)
+            for (int ns = nt + 1; ns < 3; ++ns)
+                for (int st = 0; st < 2; ++st)
+                    for (int ss = 0; ss < 2; ++ss) {
+                        const Cplx v = face_pair(G, sep, nt, st, ns, ss, q, qs, gl);
+                        const double sg = -(st == ss ? 1.0 : -1.0) / vol;
+                        A[nt][ns][0] += sg * v.r;
+                        A[nt][ns][1] += sg * v.i;
+                    }
     }
-    // xx = A_yy + A_zz, yy = A_xx + A_zz, zz = A_xx + A_yy; xy, xz, yz
     out[0] = A[1][1][0] + A[2][2][0];  out[1] = A[1][1][1] + A[2][2][1];
     out[2] = A[0][0][0] + A[2][2][0];  out[3] = A[0][0][1] + A[2][2][1];
     out[4] = A[0][0][0] + A[1][1][0];  out[5] = A[0][0][1] + A[1][1][1];
-    out[6] = A[0][1][0];  out[7] = A[0][1][1];
-    out[8] = A[0][2][0];  out[9] = A[0][2][1];
-    out[10] = A[1][2][0]; out[11] = A[1][2][1];
+    const double fx = near ? f[0] * f[1] : 1.0, fy = near ? f[0] * f[2] : 1.0, fz = near ? f[1] * f[2] : 1.0;
+    out[6] = fx * A[0][1][0];  out[7] = fx * A[0][1][1];
+    out[8] = fy * A[0][2][0];  out[9] = fy * A[0][2][1];
+    out[10] = fz * A[1][2][0]; out[11] = fz * A[1][2][1];
     if (sep[0] == 0.0 && sep[1] == 0.0 && sep[2] == 0.0) {
         // identity term -1/zeta^2 (src/glaGen.jl:128-130)
         const double z2r = G.zr * G.zr - G.zi * G.zi, z2i = 2 * G.zr * G.zi;
@@ -146,7 +381,7 @@ __device__ void green_at(const GreenCfg& G, const double sep[3], const double* _
 
 // self Toeplitz tensor over offsets >= 0: toe[(p*6 + comp)] (complex)
 __global__ void k_toeplitz(double2* __restrict__ toe, int N1, int N2, int N3, GreenCfg G,
-                           const double* __restrict__ gl)
+                           const double* __restrict__ gl, const double2* __restrict__ w18)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)N1 * N2 * N3;
@@ -154,7 +389,7 @@ __global__ void k_toeplitz(double2* __restrict__ toe, int N1, int N2, int N3, Gr
     const int a = (int)(p % N1), b = (int)((p / N1) % N2), c = (int)(p / ((int64_t)N1 * N2));
     const double sep[3] = {a * G.scl[0], b * G.scl[1], c * G.scl[2]};
     double o[12];
-    green_at(G, sep, gl, o);
+    green_at(G, sep, gl, w18, o);
     for (int k = 0; k < 6; ++k) toe[p * 6 + k] = make_double2(o[2 * k], o[2 * k + 1]);
 }
 
@@ -183,7 +418,8 @@ __global__ void k_embed_self(double2* __restrict__ crc, const double2* __restric
 // external circulant over target-minus-source separations
 // (zero for T <= n <= L - S on an axis; L = S + T is the reference's grid)
 __global__ void k_crc_ext(double2* __restrict__ crc, int T1, int T2, int T3, int S1, int S2, int S3, int L1, int L2,
-                          int L3, double d1, double d2, double d3, GreenCfg G, const double* __restrict__ gl)
+                          int L3, double d1, double d2, double d3, GreenCfg G, const double* __restrict__ gl,
+                          const double2* __restrict__ w18)
 {
     const int64_t Lt = (int64_t)L1 * L2 * L3;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,7 +432,7 @@ __global__ void k_crc_ext(double2* __restrict__ crc, int T1, int T2, int T3, int
     const double sep[3] = {d1 + (n1 < T1 ? n1 : n1 - L1) * G.scl[0], d2 + (n2 < T2 ? n2 : n2 - L2) * G.scl[1],
                            d3 + (n3 < T3 ? n3 : n3 - L3) * G.scl[2]};
     double o[12];
-    green_at(G, sep, gl, o);
+    green_at(G, sep, gl, w18, o);
     for (int k = 0; k < 6; ++k) crc[k * Lt + p] = make_double2(o[2 * k], o[2 * k + 1]);
 }
 
@@ -274,10 +510,20 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
     for (int a = 0; a < 3; ++a) G.scl[a] = rat(trg.scl_num[a], trg.scl_den[a]);
     G.zr = opts.frq_phz_re;
     G.zi = opts.frq_phz_im;
-    G.qnear = std::max(2, std::min(QMAX - 1, opts.int_ord > 0 ? opts.int_ord / 3 : 10));
     if (G.zr == 0 && G.zi == 0) {
         err = "frqPhz must be nonzero";
         return GLA_ERR_ARG;
+    }
+    {
+        const std::complex<double> z(G.zr, G.zi), c = 1.0 / (4.0 * M_PI * z * z);
+        G.kr = 2 * M_PI * G.zr;
+        G.ki = 2 * M_PI * G.zi;
+        G.cr = c.real();
+        G.ci = c.imag();
+        // a regular rule integrates the phase factor across a cell to about
+        // 1e-15 from 4 + 3 k h nodes per axis
+        const double hmax = std::max(G.scl[0], std::max(G.scl[1], G.scl[2]));
+        G.qphase = std::min(QMAX, (int)std::ceil(4.0 + 3.0 * 2.0 * M_PI * std::abs(z) * hmax));
     }
     if (!d.self_opr) {
         // overlap check of genEgoCrcExt! (src/glaGen.jl:64-79): edges of each
@@ -317,16 +563,25 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
     const int B[3] = {(int)d.dev_brn[0], (int)d.dev_brn[1], (int)d.dev_brn[2]};
     const int L[3] = {2 * B[0], 2 * B[1], 2 * B[2]};
     const int64_t Lt = (int64_t)L[0] * L[1] * L[2];
-    DevBuf dgl, dcrc, dw1, dw2, dtab, dtoe, dfur, dcnt;
+    // singular face-pair values of touching cells (DIRECTFN order intOrd)
+    std::complex<double> w18[18];
+    {
+        const gla_status st = singular_values(G.scl, G.zr, G.zi, opts.int_ord > 0 ? opts.int_ord : 32, w18, err);
+        if (st) return st;
+    }
+    DevBuf dgl, dcrc, dw1, dw2, dtab, dtoe, dfur, dcnt, dw18;
     SETUP_HIP(hipMalloc(&dgl.p, gl.size() * sizeof(double)));
     SETUP_HIP(hipMemcpy(dgl.p, gl.data(), gl.size() * sizeof(double), hipMemcpyHostToDevice));
+    SETUP_HIP(hipMalloc(&dw18.p, sizeof(w18)));
+    SETUP_HIP(hipMemcpy(dw18.p, w18, sizeof(w18), hipMemcpyHostToDevice));
     SETUP_HIP(hipMalloc(&dcrc.p, (size_t)Lt * 6 * sizeof(double2)));
     const int thr = 128;
     if (d.self_opr) {
         const int64_t Nt = d.trg_cel[0] * d.trg_cel[1] * d.trg_cel[2];
         SETUP_HIP(hipMalloc(&dtoe.p, (size_t)Nt * 6 * sizeof(double2)));
         hipLaunchKernelGGL(k_toeplitz, dim3((unsigned)((Nt + thr - 1) / thr)), dim3(thr), 0, nullptr, (double2*)dtoe.p,
-                           (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], G, (const double*)dgl.p);
+                           (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], G, (const double*)dgl.p,
+                           (const double2*)dw18.p);
         SETUP_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_embed_self, dim3((unsigned)((Lt + 255) / 256)), dim3(256), 0, nullptr, (double2*)dcrc.p,
                            (const double2*)dtoe.p, (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], L[0], L[1],
@@ -344,7 +599,7 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
         hipLaunchKernelGGL(k_crc_ext, dim3((unsigned)((Lt + thr - 1) / thr)), dim3(thr), 0, nullptr, (double2*)dcrc.p,
                            (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], (int)d.src_cel[0],
                            (int)d.src_cel[1], (int)d.src_cel[2], L[0], L[1], L[2], del[0], del[1], del[2], G,
-                           (const double*)dgl.p);
+                           (const double*)dgl.p, (const double2*)dw18.p);
         SETUP_HIP(hipGetLastError());
     }
     // forward transform over the full L grid: S = L on every axis

@@ -10,6 +10,13 @@ namespace gla {
 
 // Fourier Green data of the operator (trg <- src) in the reference's 8-branch
 // layout (src/glaMemSup.jl:138-156), computed in FP64.
+// The 18 singular face-pair values [wS(3), wE(9), wV(6)] of a cell of sizes
+// s (wavelengths), divided by the cell volume, at Gauss-Legendre order ord:
+// wekS / wekE / wekV (src/glaInt.jl:75-267) as egoFunSng! uses them
+// (src/glaGen.jl:117-121).  Runs on the current device.
+gla_status singular_values(const double s[3], double zr, double zi, int ord, std::complex<double> w18[18],
+                           std::string& err);
+
 gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& src, const gla_dims& d,
                      std::vector<std::complex<double>>& fur, std::string& err);
 

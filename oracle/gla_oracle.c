@@ -18,9 +18,11 @@
  *     egoFunExt! / grdSel (src/glaGen.jl:169-189, 632-643).
  *   - gor_brute_crc: independent direct circular sum y[t] = sum_s c[t-s] x[s]
  *     over the real-space circulant, used to pin the restatement.
- *   - gor_toe_model: test Green tensors (a point-collocation dyadic model, and
- *     the reference's face-pair surface-integral formulation evaluated with
- *     fixed-order Gauss-Legendre rules; see the function header).
+ *   - gor_toe_model: test Green tensors (a point-collocation dyadic model,
+ *     the face-pair surface-integral formulation with fixed-order rules, and
+ *     model 3, the reference's construction: egoFunInn! for separated cells
+ *     and egoFunSng! with the DIRECTFN singular values of gla_oracle_sng.c
+ *     for touching ones; see the function headers).
  *
  * How each part is pinned (tests/test_oracle.py): the FFT against the DFT
  * definition and numpy; the matvec restatement against gor_brute_crc on self,
@@ -30,10 +32,16 @@
  * linearity, reciprocity and the self-vs-external translation identity of
  * test/extSlfTest.jl:45-47; the face-pair tensor's far field against the
  * analytic dyadic of test/anaTest.jl:36-44 with the cube-averaging error
- * (kh)^2/12.  Parity unpinned: the face-pair values for touching and
- * near cells against the reference's own (adaptive hcubature and DIRECTFN
- * rules, src/glaGen.jl:350-598, not runnable here); the matvec parity does not
- * depend on them, since it runs on the same Green data on both sides.
+ * (kh)^2/12; model 3 (tests/test_oracle_singular.py) against an independent
+ * difference-coordinate quadrature of every face pair: the closed-form static
+ * panel integrals exactly, the singular values to their order-32 quadrature
+ * error (<= 1e-10 self and edge, <= 2e-9 vertex), the tensor at the eight
+ * touching offsets (the correction tables, anisotropic cells included) and at
+ * regular offsets (<= 1e-11), plus the order-32 vs order-40 convergence of
+ * test/intConTest.jl.  The reference's hcubature values of the regular face
+ * pairs (third-party, tolerance 1e-6, src/glaGen.jl:574-576) cannot be run
+ * here: model 3 integrates those pairs to their limit instead, so its
+ * agreement with the reference is bounded by that tolerance.
  *
  * Build: oracle/Makefile -> oracle/_build/libgila_oracle.so
  */
@@ -387,9 +395,149 @@ static void tensor_colloc(const double sep[3], const double s[3], zc zeta, zc ou
             out[i + 3 * j] = pre * ((i == j ? a : 0) + b * sep[i] * sep[j] / (r * r));
 }
 
+/* srfSum! (src/glaGen.jl:531-555) with 0-based srfMat indices; out is the
+ * column-major 3x3 tensor */
+static void srf_sum(const zc *m, zc out[9])
+{
+    out[0] = m[14] - m[15] - m[20] + m[21] + m[28] - m[29] - m[34] + m[35];
+    out[1] = -m[12] + m[13] + m[18] - m[19];
+    out[2] = -m[24] + m[25] + m[30] - m[31];
+    out[3] = -m[2] + m[3] + m[8] - m[9];
+    out[4] = m[0] - m[1] - m[6] + m[7] + m[28] - m[29] - m[34] + m[35];
+    out[5] = -m[26] + m[27] + m[32] - m[33];
+    out[6] = -m[4] + m[5] + m[10] - m[11];
+    out[7] = -m[16] + m[17] + m[22] - m[23];
+    out[8] = m[0] - m[1] - m[6] + m[7] + m[14] - m[15] - m[20] + m[21];
+}
+
+/* one face pair (target face ft, source face fs, facPar order) over [0,1]^4
+ * with an order-q tensor Gauss-Legendre rule, times srfScl: the egoSrfAdp!
+ * value (src/glaGen.jl:559-598) with a fixed rule instead of hcubature */
+static zc pair_gl(const double F[6][4][3], int ft, int fs, const double sep[3], const double s[3], zc zeta,
+                  int q, const double *gx, const double *gw)
+{
+    const double scale = ((ft < 2) ? s[1] * s[2] : (ft < 4 ? s[0] * s[2] : s[0] * s[1])) /
+                         ((fs < 2) ? s[0] : (fs < 4 ? s[1] : s[2]));
+    double et1[3], et2[3], es1[3], es2[3];
+    for (int d = 0; d < 3; ++d) {
+        et1[d] = F[ft][1][d] - F[ft][0][d]; et2[d] = F[ft][3][d] - F[ft][0][d];
+        es1[d] = F[fs][1][d] - F[fs][0][d]; es2[d] = F[fs][3][d] - F[fs][0][d];
+    }
+    zc acc = 0;
+    for (int a = 0; a < q; ++a)
+        for (int b = 0; b < q; ++b)
+            for (int c = 0; c < q; ++c)
+                for (int e = 0; e < q; ++e) {
+                    double r2 = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        double dv = (F[ft][0][d] + gx[c] * et1[d] + gx[e] * et2[d]) -
+                                    (F[fs][0][d] + gx[a] * es1[d] + gx[b] * es2[d]) + sep[d];
+                        r2 += dv * dv;
+                    }
+                    double r = sqrt(r2);
+                    zc g = cexp(2.0 * M_PI * I * r * zeta) / (4.0 * M_PI * r * zeta * zeta);
+                    acc += gw[a] * gw[b] * gw[c] * gw[e] * g;
+                }
+    return acc * scale;
+}
+
+/* Model 3, the reference's construction: egoFunInn! (all 36 face pairs,
+ * src/glaGen.jl:322-338) for cells more than one cell apart on some axis, and
+ * egoFunSng! (src/glaGen.jl:350-525) for offsets in {0,1}^3: the face pairs
+ * its mask marks take the singular panel values wS / wE / wV divided by the
+ * cell volume (src/glaGen.jl:117-121), the others the regular integral.
+ * Regular pairs use an order-q tensor Gauss-Legendre rule, q from the
+ * separation in cells and the cell size in wavelengths (converged to about
+ * 1e-14 relative for every pair; the reference's hcubature stops at 1e-6).
+ * A separation with a negative component in {-1, 0, 1} cells takes the value
+ * at its absolute offset with the component flips f_i f_j, as the contact
+ * path of the reference does through the contact volume's circulant
+ * (egoCntOut!, src/glaGen.jl:250-304). */
+enum { S1_ = 1, S2_, S3_, E1_, E2_, E3_, E4_, E5_, E6_, E7_, E8_, E9_, V1_, V2_, V3_, V4_, V5_, V6_ };
+#define __ 0
+/* [offset o1 + 2 o2 + 4 o3][target face][source face], faces yzL yzU xzL
+ * xzU xyL xyU: the corVal tables of egoFunSng! (mask = nonzero entry) */
+static const unsigned char COR[8][6][6] = {
+    /* (0,0,0) */
+    {{S1_, __, E7_, E7_, E8_, E8_}, {__, S1_, E7_, E7_, E8_, E8_}, {E7_, E7_, S2_, __, E9_, E9_},
+     {E7_, E7_, __, S2_, E9_, E9_}, {E8_, E8_, E9_, E9_, S3_, __}, {E8_, E8_, E9_, E9_, __, S3_}},
+    /* (1,0,0) */
+    {{__, S1_, E7_, E7_, E8_, E8_}, {__, __, __, __, __, __}, {__, E7_, E3_, __, V6_, V6_},
+     {__, E7_, __, E3_, V6_, V6_}, {__, E8_, V6_, V6_, E5_, __}, {__, E8_, V6_, V6_, __, E5_}},
+    /* (0,1,0) */
+    {{E1_, __, __, E7_, V5_, V5_}, {__, E1_, __, E7_, V5_, V5_}, {E7_, E7_, __, S2_, E9_, E9_},
+     {__, __, __, __, __, __}, {V5_, V5_, __, E9_, E6_, __}, {V5_, V5_, __, E9_, __, E6_}},
+    /* (1,1,0) */
+    {{__, E1_, __, E7_, V5_, V5_}, {__, __, __, __, __, __}, {__, E7_, __, E3_, V6_, V6_},
+     {__, __, __, __, __, __}, {__, V5_, __, V6_, V3_, __}, {__, V5_, __, V6_, __, V3_}},
+    /* (0,0,1) */
+    {{E2_, __, V4_, V4_, __, E8_}, {__, E2_, V4_, V4_, __, E8_}, {V4_, V4_, E4_, __, __, E9_},
+     {V4_, V4_, __, E4_, __, E9_}, {E8_, E8_, E9_, E9_, __, S3_}, {__, __, __, __, __, __}},
+    /* (1,0,1) */
+    {{__, E2_, V4_, V4_, __, E8_}, {__, __, __, __, __, __}, {__, V4_, V2_, __, __, V6_},
+     {__, V4_, __, V2_, __, V6_}, {__, E8_, V6_, V6_, __, E5_}, {__, __, __, __, __, __}},
+    /* (0,1,1) */
+    {{V1_, __, __, V4_, __, V5_}, {__, V1_, __, V4_, __, V5_}, {V4_, V4_, __, E4_, __, E9_},
+     {__, __, __, __, __, __}, {V5_, V5_, __, E9_, __, E6_}, {__, __, __, __, __, __}},
+    /* (1,1,1) */
+    {{__, V1_, __, V4_, __, V5_}, {__, __, __, __, __, __}, {__, V4_, __, V2_, __, V6_},
+     {__, __, __, __, __, __}, {__, V5_, __, V6_, __, V3_}, {__, __, __, __, __, __}}};
+#undef __
+
+static zc g_w18[18];   /* [wS(3) wE(9) wV(6)] / cell volume, set by gor_set_singular */
+
+void gor_set_singular(const zc *w18) { memcpy(g_w18, w18, sizeof(g_w18)); }
+
+static int ref_order(const double sep[3], const double s[3], zc zeta)
+{
+    double D = 0, hm = 0;
+    for (int d = 0; d < 3; ++d) {
+        D = fmax(D, fabs(sep[d]) / s[d]);
+        hm = fmax(hm, s[d]);
+    }
+    int q = D <= 2.5 ? 12 : (D <= 4.5 ? 10 : (D <= 8.5 ? 8 : 6));
+    const int qp = 6 + (int)ceil(6.0 * 2.0 * M_PI * cabs(zeta) * hm);
+    return q > qp ? q : (qp > 20 ? 20 : qp);
+}
+
+static void tensor_ref(const double sep[3], const double s[3], zc zeta, zc out[9])
+{
+    double F[6][4][3];
+    cub_fac(s, F);
+    int near = 1, o[3];
+    double f[3], asep[3];
+    for (int d = 0; d < 3; ++d) {
+        const double t = sep[d] / s[d];
+        const long r = lround(t);
+        if (fabs(t - (double)r) > 1e-9 || labs(r) > 1) near = 0;
+        o[d] = (int)labs(r);
+        f[d] = r < 0 ? -1.0 : 1.0;
+        asep[d] = (double)o[d] * s[d];
+    }
+    const int q = ref_order(sep, s, zeta);
+    double gx[24], gw[24];
+    gl_rule(q, gx, gw);
+    zc srf[36];
+    for (int fp = 0; fp < 36; ++fp) {
+        const int ft = fp / 6, fs = fp % 6;
+        const int code = near ? COR[o[0] + 2 * o[1] + 4 * o[2]][ft][fs] : 0;
+        srf[fp] = code ? g_w18[code - 1] : pair_gl(F, ft, fs, near ? asep : sep, s, zeta, q, gx, gw);
+    }
+    srf_sum(srf, out);
+    if (near)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                if (i != j) out[i + 3 * j] *= f[i] * f[j];
+    if (sep[0] == 0 && sep[1] == 0 && sep[2] == 0) {
+        zc id = 1.0 / (zeta * zeta);
+        out[0] -= id; out[4] -= id; out[8] -= id;
+    }
+}
+
 static void tensor_at(int model, const double sep[3], const double s[3], zc zeta, int qf, int qn, zc out[9])
 {
-    if (model == 1 || model == 2) tensor_facepair(sep, s, zeta, qf, qn, model == 2, out);
+    if (model == 3) tensor_ref(sep, s, zeta, out);
+    else if (model == 1 || model == 2) tensor_facepair(sep, s, zeta, qf, qn, model == 2, out);
     else tensor_colloc(sep, s, zeta, out);
 }
 

@@ -40,10 +40,17 @@ def lib():
             "gor_crc_ext": [i32, vp, vp, vp, vp, d, d, i32, i32, vp],
             "gor_fft_axis_c128": [vp, vp, i64, i32, i32],
             "gor_gl_rule": [i32, vp, vp],
+            "gor_gl_pm1": [i32, vp, vp],
+            "gor_wek_s": [vp, d, d, i32, vp],
+            "gor_wek_e": [vp, d, d, i32, vp],
+            "gor_wek_v": [vp, d, d, i32, vp],
+            "gor_r_srf": [i32, d, d, d, vp],
         }.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
+        L.gor_set_singular.argtypes = [vp]
+        L.gor_set_singular.restype = None
         L.gor_set_threads.argtypes = [i32]
         L.gor_set_threads.restype = None
         L.gor_num_threads.restype = ctypes.c_int
@@ -78,9 +85,32 @@ def dims(S, T, sym):
     return tuple(int(b) for b in B), tuple(int(u) for u in U)
 
 
-def toe_model(N, scl, model=0, zeta=1.0 + 0j, qf=4, qn=8):
+_SNG = {}
+
+
+def singular_values(scl, zeta=1.0 + 0j, order=32):
+    """[wS(3), wE(9), wV(6)] divided by the cell volume, as egoFunSng! uses
+    them (src/glaGen.jl:117-121); cached per (scl, zeta, order)."""
+    key = (tuple(float(v) for v in scl), complex(zeta), int(order))
+    if key not in _SNG:
+        vol = float(np.prod(scl))
+        _SNG[key] = np.concatenate([wek("S", scl, zeta, order), wek("E", scl, zeta, order),
+                                    wek("V", scl, zeta, order)]) / vol
+    return _SNG[key]
+
+
+def _use_singular(model, scl, zeta, order):
+    if model == 3:
+        w = np.ascontiguousarray(singular_values(scl, zeta, order))
+        lib().gor_set_singular(_p(w))
+
+
+def toe_model(N, scl, model=0, zeta=1.0 + 0j, qf=4, qn=8, int_ord=32):
     """Real-space Toeplitz tensor (3,3,N1,N2,N3) of a test Green model
-    (0: point collocation, 1: face-pair surface integrals)."""
+    (0: point collocation, 1/2: face-pair surface integrals with fixed rules,
+    3: the reference's construction with the DIRECTFN singular values of
+    order int_ord)."""
+    _use_singular(model, scl, zeta, int_ord)
     toe = np.zeros((3, 3) + tuple(N), dtype=np.complex128, order="F")
     lib().gor_toe_model(int(model), _p(_i64(N)), _p(np.asarray(scl, np.float64)), zeta.real, zeta.imag,
                         int(qf), int(qn), _p(toe))
@@ -93,7 +123,8 @@ def crc_self(N, toe):
     return crc
 
 
-def crc_ext(S, T, scl, delta, model=0, zeta=1.0 + 0j, qf=4, qn=8):
+def crc_ext(S, T, scl, delta, model=0, zeta=1.0 + 0j, qf=4, qn=8, int_ord=32):
+    _use_singular(model, scl, zeta, int_ord)
     crc = np.zeros((3, 3) + tuple(s + t for s, t in zip(S, T)), dtype=np.complex128, order="F")
     if lib().gor_crc_ext(int(model), _p(_i64(S)), _p(_i64(T)), _p(np.asarray(scl, np.float64)),
                          _p(np.asarray(delta, np.float64)), zeta.real, zeta.imag, int(qf), int(qn), _p(crc)):
@@ -152,6 +183,32 @@ def gl_rule(n):
     return x, w
 
 
+def wek(kind, scl, zeta=1.0 + 0j, order=32):
+    """The reference's singular panel values (wekS / wekE / wekV,
+    src/glaInt.jl:75-267) for cell sizes scl (wavelengths), before the
+    division by the cell volume: 3, 9 or 6 complex values."""
+    n = {"S": 3, "E": 9, "V": 6}[kind]
+    out = np.zeros(n, np.complex128)
+    fn = {"S": lib().gor_wek_s, "E": lib().gor_wek_e, "V": lib().gor_wek_v}[kind]
+    fn(_p(np.asarray(scl, np.float64)), zeta.real, zeta.imag, int(order), _p(out))
+    return out
+
+
+def r_srf(which, la, lb, lc=0.0):
+    """Closed-form static panel integrals at zeta = 1: which 0 rSrfSlf(la, lb),
+    1 rSrfEdgFlt(la, lb), 2 rSrfEdgCrn(la, lb, lc) (src/glaInt.jl:410-488)."""
+    out = np.zeros(2)
+    lib().gor_r_srf(int(which), float(la), float(lb), float(lc), _p(out))
+    return complex(out[0], out[1])
+
+
+def gl_pm1(n):
+    """Gauss-Legendre nodes and weights on [-1, 1], ascending (gauQud)."""
+    x, w = np.zeros(n), np.zeros(n)
+    lib().gor_gl_pm1(int(n), _p(x), _p(w))
+    return x, w
+
+
 # ---------------------------------------------------------------------------
 # portable seeded inputs: SplitMix64 -> uniform doubles -> Box-Muller
 # ---------------------------------------------------------------------------
@@ -181,9 +238,10 @@ def complex_gaussian(shape, seed):
     return np.asfortranarray(g.reshape(shape, order="F"))
 
 
-def self_operator_fur(N, h=1.0 / 32, model=0, zeta=1.0 + 0j, qf=4, qn=8):
+def self_operator_fur(N, h=1.0 / 32, model=0, zeta=1.0 + 0j, qf=4, qn=8, int_ord=32):
     """Fourier branches of a self operator on N cells from a test Green model."""
-    toe = toe_model(N, (h, h, h), model=model, zeta=zeta, qf=qf, qn=qn)
+    hs = tuple(h) if np.ndim(h) else (h, h, h)
+    toe = toe_model(N, hs, model=model, zeta=zeta, qf=qf, qn=qn, int_ord=int_ord)
     crc = crc_self(N, toe)
     return fur_from_crc(N, N, (1, 1, 1), crc), crc
 
