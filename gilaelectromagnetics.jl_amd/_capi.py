@@ -29,7 +29,7 @@ GLA_NOALLOC_SCRATCH_BYTES = 256 << 20
 EXPORTED = (
     "gla_last_error", "gla_init", "gla_opr_dims_for", "gla_opr_create_from_fur", "gla_opr_create",
     "gla_opr_export_fur", "gla_opr_get_dims", "gla_opr_adjoint", "gla_opr_destroy", "gla_opr_apply",
-    "gla_opr_apply_axpby", "gla_opr_apply_host", "gla_opr_apply_noalloc", "gla_opr_workspace_bytes", "gla_opr_profile",
+    "gla_opr_apply_axpby", "gla_opr_apply_host", "gla_opr_apply_noalloc", "gla_opr_reserve", "gla_opr_workspace_bytes", "gla_opr_profile",
     "gla_opr_profile_seq",
     "gla_ls_apply", "gla_krylov_dots_workspace", "gla_krylov_dots", "gla_krylov_axpy",
     "gla_krylov_rows", "gla_krylov_next", "gla_krylov_givens", "gla_opr_get_vols", "gla_opr_get_opts", "gla_opr_save", "gla_opr_load",
@@ -87,6 +87,7 @@ def load(path=LIB_PATH):
         "gla_opr_apply_axpby": [vp, vp, i64, vp, i64, i64, d, d, d, d, vp],
         "gla_opr_apply_noalloc": [vp, vp, i64, vp, i64, i64, vp],
         "gla_opr_apply_host": [vp, vp, i64, vp, i64, i64],
+        "gla_opr_reserve": [vp, i64],
         "gla_opr_profile": [vp, vp, i64, vp, i64, i64, vp, i32, P(d), P(i64)],
         "gla_opr_profile_seq": [vp, vp, i64, vp, i64, i64, vp, i32, P(d), P(i64), P(d)],
         "gla_ls_apply": [vp, vp, i64, i64, vp, i64, vp, i64, i64, vp],

@@ -330,6 +330,15 @@ gla_status run(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int6
     if (ny != h->geom.ys * nrhs) return fail(GLA_ERR_SIZE, "Size of output vector does not match the target volume.");
     std::lock_guard<std::mutex> lk(h->mu);
     DevGuard g(h->opts.device);
+    if (h->ws_nrhs < nrhs) {
+        // growing the work buffers frees and allocates, which a stream under
+        // graph capture must not see
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (stream && hipStreamIsCapturing((hipStream_t)stream, &cs) == hipSuccess &&
+            cs != hipStreamCaptureStatusNone)
+            return fail(GLA_ERR_ARG, "work buffers for " + std::to_string(nrhs) +
+                                         " right-hand sides must be reserved (gla_opr_reserve) before capture");
+    }
     gla_status s = ensure_ws(h, nrhs);
     if (s) return s;
     hipError_t e;
@@ -871,6 +880,15 @@ gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, voi
 }  // namespace
 
 extern "C" {
+
+gla_status gla_opr_reserve(gla_opr_t h, int64_t nrhs)
+{
+    if (!h) return fail(GLA_ERR_ARG, "null operator");
+    if (nrhs < 1 || nrhs > 65535) return fail(GLA_ERR_ARG, "nrhs must be in [1, 65535]");
+    std::lock_guard<std::mutex> lk(h->mu);
+    DevGuard g(h->opts.device);
+    return ensure_ws(h, nrhs);
+}
 
 int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs)
 {

@@ -222,6 +222,12 @@ gla_status gla_opr_get_opts(gla_opr_t h, gla_opts* opts);
 gla_status gla_opr_save(gla_opr_t h, const char* path);
 gla_status gla_opr_load(const char* path, int32_t dtype, int32_t device, gla_opr_t* out);
 
+/* Grow the handle's work buffers to nrhs right-hand sides now.  An apply on
+ * a stream under HIP graph capture must not allocate, so it fails with
+ * GLA_ERR_ARG instead of growing them: reserve before capturing.  No
+ * counterpart in the reference (its device path allocates per call). */
+gla_status gla_opr_reserve(gla_opr_t h, int64_t nrhs);
+
 /* Bytes of device work memory the handle holds for nrhs right-hand sides. */
 int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs);
 
