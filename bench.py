@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-external", action="store_true", help="skip the external-operator leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched right-hand-side leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served solve-stream leg")
@@ -353,6 +353,76 @@ def serve_leg(gila, a, log):
             "sweep": sweep}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(a, ge, N, x, r32, r64):
+    """The CPU oracle -- the C restatement of the reference's host matvec
+    egoBrnHst! (oracle/gla_oracle.c), OpenMP over all the host threads it is
+    given -- timed on this host: the headline 128^3 ComplexF32 workload on all
+    threads (median of --cpu-reps after one warm-up), the same in ComplexF64,
+    the 128^3 ComplexF32 matvec on one thread (3 runs), and the reference's
+    CPU configuration, a 16^3 ComplexF64 self operator (all threads and one
+    thread, median of --cpu-reps each).  Returns (cpu_baseline, parity)."""
+    orc = ge.load_oracle()     # the checker: CPU baseline and parity only
+    sym = (1, 1, 1)
+    fur64 = np.concatenate([f.reshape(-1, order="F") for f in r64["mem"].egoFur])
+    fur32 = np.concatenate([f.reshape(-1, order="F") for f in r32["mem"].egoFur])
+    x32 = x.astype(np.complex64)
+    nthr = orc.num_threads()
+
+    def timed(fn, reps):
+        fn()   # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+    log("cpu leg: FP64 oracle matvec (parity reference)")
+    y_ref = orc.matvec(N, N, sym, fur64, x)
+    log(f"cpu leg: 128^3 ComplexF32 on {nthr} threads")
+    t32 = timed(lambda: orc.matvec(N, N, sym, fur32, x32, dtype=np.complex64), a.cpu_reps)
+    log(f"cpu leg: 128^3 ComplexF64 on {nthr} threads")
+    t64 = timed(lambda: orc.matvec(N, N, sym, fur64, x), a.cpu_reps)
+    log("cpu leg: 128^3 ComplexF32 on 1 thread")
+    orc.set_threads(1)
+    try:
+        t32_1 = timed(lambda: orc.matvec(N, N, sym, fur32, x32, dtype=np.complex64), 3)
+        n16 = (16, 16, 16)
+        fur16, _ = orc.self_operator_fur(n16, model=0)
+        x16 = orc.complex_gaussian(n16 + (3,), seed=1)
+        t16_1 = timed(lambda: orc.matvec(n16, n16, sym, fur16, x16), a.cpu_reps)
+    finally:
+        orc.set_threads(nthr)
+    t16 = timed(lambda: orc.matvec(n16, n16, sym, fur16, x16), a.cpu_reps)
+    try:
+        online = len(os.sched_getaffinity(0))
+    except AttributeError:
+        online = os.cpu_count()
+    base = {"value": 1.0 / t32, "unit": "matvecs/s", "cores": nthr, "kind": "port",
+            "sample": f"128^3 ComplexF32 self matvec, the same workload and Green data as the GPU headline: C "
+                      f"restatement of egoBrnHst! (oracle/gla_oracle.c), median of {a.cpu_reps} after one warm-up, "
+                      f"{t32:.3f} s per matvec",
+            "cpu_model": cpu_model(), "host_cores_online": online,
+            "c128_128": {"matvecs_per_s": 1.0 / t64, "s_per_matvec": t64, "threads": nthr},
+            "c64_128_one_thread": {"matvecs_per_s": 1.0 / t32_1, "s_per_matvec": t32_1, "threads": 1},
+            "c128_16_reference_cpu_config": {"s_per_matvec": t16, "threads": nthr, "s_per_matvec_one_thread": t16_1,
+                                             "workload": "16^3 self operator of 1/32-wavelength cells, ComplexF64, "
+                                                         "seeded complex Gaussian input"}}
+    parity = {"c64_rel_l2_vs_fp64_oracle": orc.rel_l2(r32["y"], y_ref),
+              "c128_rel_l2_vs_fp64_oracle": orc.rel_l2(r64["y"], y_ref)}
+    return base, parity
+
+
 def main():
     a = parse()
     if a.gpus > 1:
@@ -439,27 +509,7 @@ def main():
         out["roofline"]["traffic"] = tr.get(dom["kernel"])
 
     if not a.no_cpu:
-        log("cpu leg: FP64 oracle matvec")
-        orc = ge.load_oracle()     # the checker: CPU baseline and parity only
-        fur64 = np.concatenate([f.reshape(-1, order="F") for f in r64["mem"].egoFur])
-        sym = (1, 1, 1)
-        y_ref = orc.matvec(N, N, sym, fur64, x)
-        fur32 = np.concatenate([f.reshape(-1, order="F") for f in r32["mem"].egoFur])
-        x32 = x.astype(np.complex64)
-        orc.matvec(N, N, sym, fur32, x32, dtype=np.complex64)   # warm-up
-        ts = []
-        log("cpu leg: timing the FP32 restatement")
-        for _ in range(a.cpu_reps):
-            t0 = time.perf_counter()
-            orc.matvec(N, N, sym, fur32, x32, dtype=np.complex64)
-            ts.append(time.perf_counter() - t0)
-        med = float(np.median(ts))
-        out["cpu_baseline"] = {"value": 1.0 / med, "unit": "matvecs/s", "cores": orc.num_threads(), "kind": "port",
-                               "sample": f"{a.cpu_reps} matvecs (after 1 warm-up) of the same 128^3 ComplexF32 "
-                                         f"workload, C restatement of egoBrnHst! (oracle/gla_oracle.c), median "
-                                         f"{med:.3f} s"}
-        out["parity"] = {"c64_rel_l2_vs_fp64_oracle": orc.rel_l2(r32["y"], y_ref),
-                         "c128_rel_l2_vs_fp64_oracle": orc.rel_l2(r64["y"], y_ref)}
+        out["cpu_baseline"], out["parity"] = cpu_leg(a, ge, N, x, r32, r64)
     del res, r32, r64
     torch.cuda.empty_cache()
     if not a.no_external:

@@ -177,3 +177,29 @@ def test_server_queued_steps_on_gpu(gila, orc, gpu):
         assert np.linalg.norm(a.result - b.result) <= 1e-6 * np.linalg.norm(a.result)
         fur = np.concatenate([f.reshape(-1, order="F") for f in srv.ops[b.key].mem64.egoFur])
         assert _residual(orc, fur, b) <= 2e-6
+
+
+@pytest.mark.gpu
+def test_server_cfg5_domains_alone_and_in_the_mix(gila, orc, gpu):
+    """Served-stream requests on the stream's own domain sizes (32^3,
+    64x64x48, 128^3; 1/32-wavelength cells, chi in a random 10-40% box, plane
+    wave): every result converged by the FP64 residual and checked by the
+    FP64 oracle operator built from the same Green data (<= 1e-6 plus the
+    rounding of the two FP64 residual evaluations), and the 128^3 request
+    solved alone equals its result in the mix (FP32 inner solves, FP64
+    refinement: agreement to 1e-5)."""
+    shapes = [(32, 32, 32), (64, 64, 48), (128, 128, 128), (32, 32, 32), (64, 64, 48)]
+    print("case serve cfg5 domains " + ", ".join("x".join(map(str, s)) for s in shapes), flush=True)
+    reqs = [gila.serve.random_request(20000 + i, c, tol=1e-6) for i, c in enumerate(shapes)]
+    srv = gila.serve.Server(max_batch=16, max_cycles=60)
+    done = srv.run(reqs, arrivals=[0.0, 0.0, 0.05, 0.1, 0.1], timeout=240)
+    assert all(r.converged for r in done)
+    for r in reqs:
+        fur = np.concatenate([f.reshape(-1, order="F") for f in srv.ops[r.key].mem64.egoFur])
+        res = _residual(orc, fur, r)
+        print(f"  {'x'.join(map(str, r.cells))}: steps={r.steps} relres={r.relres:.2e} oracle={res:.2e}", flush=True)
+        assert res <= 1.05e-6
+    target = gila.serve.random_request(20002, (128, 128, 128), tol=1e-6)
+    alone = gila.serve.Server(max_batch=16, max_cycles=60).run([target], timeout=240)[0]
+    assert alone.converged
+    assert np.linalg.norm(alone.result - reqs[2].result) <= 1e-5 * np.linalg.norm(alone.result)
