@@ -54,10 +54,13 @@ def parse():
     ap.add_argument("--no-external", action="store_true", help="skip the external-operator leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched right-hand-side leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the served solve-stream leg")
-    ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-rates", default="2,4,8",
-                    help="Poisson arrival rates of the served stream (comma-separated), in multiples of the "
-                         "measured sequential FP64 rate; the last is the headline")
+    ap.add_argument("--serve-requests", type=int, default=500,
+                    help="requests of the served stream at each rate (a rate may carry its own count, rate:count)")
+    ap.add_argument("--serve-seq-requests", type=int, default=40,
+                    help="bounded sample of the stream solved one at a time in plain FP64 (the baseline rate)")
+    ap.add_argument("--serve-rates", default="4",
+                    help="Poisson arrival rates of the served stream (comma-separated, each optionally rate:count), "
+                         "in multiples of the measured sequential FP64 rate; the last is the headline")
     ap.add_argument("--counters", choices=["c64", "c128"], default=None,
                     help="short program for rocprofv3 counter passes: a 512 MiB device copy (the calibration "
                          "of FETCH_SIZE/WRITE_SIZE) and three matvecs of the given type; prints no JSON")
@@ -250,10 +253,24 @@ def batched_leg(gila, log):
     return out
 
 
+def model_bytes(S, T, sym, s):
+    """Algorithmic bytes of one matvec of the pruned five-pass pipeline on the
+    reference's grid L = S + T (not a longer device grid a transform length
+    is embedded in): s [3 (S + 2 L1 S2 S3 + 2 L1 L2 S3 + 2 L1 L2 T3 +
+    2 L1 T2 T3 + T) + 6 U1 U2 U3], U = S + 1 on a symmetric axis, else L."""
+    L = [a + b for a, b in zip(S, T)]
+    U = [S[d] + 1 if sym[d] else L[d] for d in range(3)]
+    f = (S[0] * S[1] * S[2] + 2 * L[0] * S[1] * S[2] + 2 * L[0] * L[1] * S[2] + 2 * L[0] * L[1] * T[2]
+         + 2 * L[0] * T[1] * T[2] + T[0] * T[1] * T[2])
+    return s * (3 * f + 6 * U[0] * U[1] * U[2])
+
+
 def external_leg(gila, log, steps=20, warmup=50):
     """External operator with uneven volumes: a 256x34x34 source centred in a
-    256x256x128 target at 1/32 wavelength (B = (256,145,81); axes 2 and 3 run
-    on the next compiled lengths 160 and 96).  Input: an x-polarised point
+    256x256x128 target at 1/32 wavelength (B = (256,145,81); built from
+    geometry, axes 2 and 3 run on the next fast transform lengths 160 and 96,
+    while bytes_alg_per_matvec and the GB/s count the bytes of the
+    reference's 512x290x162 grid).  Input: an x-polarised point
     dipole in a seeded random source cell plus 1e-3 complex Gaussian noise.
     ComplexF32 and ComplexF64, Green data from the library's FP64 setup; the
     same two timed regions as the headline matvec."""
@@ -277,10 +294,12 @@ def external_leg(gila, log, steps=20, warmup=50):
         xd = torch.from_numpy(np.asfortranarray(x.astype(typ)).ravel(order="F").copy()).cuda()
         yd = torch.empty(3 * T[0] * T[1] * T[2], dtype=xd.dtype, device=xd.device)
         ms, _, prof, _ = time_matvec(gila, mem, xd, yd, steps, warmup)
-        by = sum(q["bytes"] for q in prof)
+        dev_by = sum(q["bytes"] for q in prof)
+        by = model_bytes(S, T, mem._dims.sym, np.dtype(typ).itemsize)
         out[key] = {"ms_per_matvec": ms, "matvecs_per_s": 1000.0 / ms, "bytes_alg_per_matvec": by,
                     "hbm_gbs_whole_matvec": by / (ms * 1e6), "branch_lengths": list(mem._dims.brn),
-                    "device_lengths": list(mem._dims.dev_brn), "green_setup_s": setup_s, "passes": prof}
+                    "device_lengths": list(mem._dims.dev_brn), "device_grid_bytes_per_matvec": dev_by,
+                    "green_setup_s": setup_s, "passes": prof}
         log(f"external {key}: {ms:.4f} ms/matvec")
         del mem, xd, yd
         torch.cuda.empty_cache()
@@ -289,62 +308,71 @@ def external_leg(gila, log, steps=20, warmup=50):
 
 
 def serve_leg(gila, a, log):
-    """Served solve stream against sequential plain FP64 solves of the same
-    requests (GMRES(30) on (I - chi G) E = E_inc to 1e-6 by the FP64
-    residual): the sequential rate first, then the same requests through the
-    server with Poisson arrivals at each rate of the sweep (one warm server,
-    operators resident across the sweep)."""
+    """Served solve stream against sequential plain FP64 solves (GMRES(30) on
+    (I - chi G) E = E_inc to 1e-6 by the FP64 residual): the sequential rate
+    from a bounded sample of the stream solved one at a time, then the stream
+    through the server with Poisson arrivals at each rate of the sweep (one
+    warm server, operators resident across the sweep)."""
     S = gila.serve
     shapes = [(32, 32, 32), (64, 64, 48), (128, 64, 64), (128, 128, 128), (256, 128, 64)]
     from gila_amd import inputs
-    rates = [float(v) for v in a.serve_rates.split(",") if v.strip()]
-    u = inputs.uniform(31337, a.serve_requests * (1 + len(rates)))
+    rates = []
+    for tok in a.serve_rates.split(","):
+        if tok.strip():
+            r, _, n = tok.partition(":")
+            rates.append((float(r), int(n) if n else a.serve_requests))
+    nmax = max([n for _, n in rates] + [a.serve_seq_requests])
+    u = inputs.uniform(31337, nmax * (1 + len(rates)))
 
-    def requests():
-        return [S.random_request(20000 + i, shapes[int(u[i] * len(shapes))], tol=1e-6)
-                for i in range(a.serve_requests)]
+    def requests(n):
+        return [S.random_request(20000 + i, shapes[int(u[i] * len(shapes))], tol=1e-6) for i in range(n)]
     log(f"serve: operators for {len(shapes)} grids (setup, not timed)")
     seq = S.Server(inner="fp64", max_batch=1, max_active=1, max_cycles=60)
     srv = S.Server(inner="fp32", max_batch=16, max_cycles=60)
     for sh in shapes:
         seq._grid_for(S.random_request(1, sh))
         srv._grid_for(S.random_request(1, sh))
-    reqs = requests()
-    log(f"serve: {len(reqs)} sequential plain FP64 solves")
+    reqs = requests(a.serve_seq_requests)
+    log(f"serve: {len(reqs)} sequential plain FP64 solves (the first requests of the stream)")
     t0 = time.perf_counter()
     seq.run(reqs, on_done=lambda r: log(f"  seq {r.rid} {'x'.join(map(str, r.cells))} steps={r.steps}"))
     t_seq = time.perf_counter() - t0
     rate_seq = len(reqs) / t_seq
     st_seq = S.latency_stats(reqs)
     sweep = []
-    for k, mult in enumerate(rates):
+    for k, (mult, nreq) in enumerate(rates):
         lam = mult * rate_seq
-        uu = u[a.serve_requests * (1 + k):a.serve_requests * (2 + k)]
+        uu = u[nmax * (1 + k):nmax * (1 + k) + nreq]
         gaps = -np.log(uu) / lam
         arrivals = np.cumsum(gaps) - gaps[0]
-        reqs2 = requests()
-        log(f"serve: served stream, Poisson arrivals at {lam:.2f}/s ({mult}x the sequential rate)")
+        reqs2 = requests(nreq)
+        log(f"serve: {nreq} requests, Poisson arrivals at {lam:.2f}/s ({mult}x the sequential rate)")
         srv.run(reqs2, arrivals=list(arrivals),
                 on_done=lambda r: log(f"  srv {r.rid} {'x'.join(map(str, r.cells))} steps={r.steps} "
                                       f"lat={r.latency:.2f}s"))
         st = S.latency_stats(reqs2)
+        # the requests solved both ways agree (FP32 inner solves with FP64
+        # refinement against plain FP64)
         agree = max(float(np.linalg.norm(r2.result - r1.result) / np.linalg.norm(r1.result))
                     for r1, r2 in zip(reqs, reqs2))
-        sweep.append({"rate_x_sequential": mult, "arrival_rate_per_s": lam, "solves_per_s": st["solves_per_s"],
+        sweep.append({"rate_x_sequential": mult, "requests": nreq, "arrival_rate_per_s": lam,
+                      "solves_per_s": st["solves_per_s"],
                       "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
                       "speedup_vs_sequential_fp64": st["solves_per_s"] / rate_seq,
                       "max_rel_diff_served_vs_sequential": agree,
                       "all_converged": bool(all(r.converged for r in reqs2)),
                       "max_relres": max(r.relres for r in reqs2), "steps_total": sum(r.steps for r in reqs2)})
     head = sweep[-1]
-    return {"workload": f"{len(reqs)} requests, domains drawn (seeded) from "
+    return {"workload": f"{head['requests']} requests, domains drawn (seeded) from "
                         + ", ".join("x".join(map(str, s)) for s in shapes)
                         + " cells at 1/32 wavelength; chi 0 outside a random box of 10-40% of the domain, "
                           "U[0.5,3]+i U[0,0.1] inside; E_inc a plane wave at a random angle; Poisson arrivals "
                           "at each rate of the sweep (headline: the last)",
+            "requests": head["requests"],
             "solves_per_s": head["solves_per_s"], "p50_latency_s": head["p50_latency_s"],
             "p99_latency_s": head["p99_latency_s"], "arrival_rate_per_s": head["arrival_rate_per_s"],
-            "sequential_fp64_solves_per_s": rate_seq, "sequential_fp64_p50_latency_s": st_seq["p50_s"],
+            "sequential_fp64_solves_per_s": rate_seq, "sequential_fp64_sample_requests": len(reqs),
+            "sequential_fp64_p50_latency_s": st_seq["p50_s"],
             "sequential_fp64_all_converged": bool(all(r.converged for r in reqs)),
             "speedup_vs_sequential_fp64": head["speedup_vs_sequential_fp64"],
             "max_rel_diff_served_vs_sequential": max(r["max_rel_diff_served_vs_sequential"] for r in sweep),

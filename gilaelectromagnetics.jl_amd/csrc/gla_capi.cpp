@@ -162,7 +162,16 @@ gla_status gla_init(int32_t device)
     return scratch_addr(device, &p);
 }
 
-gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* out)
+}  // extern "C"
+
+namespace {
+
+// exact: the layout of Fourier data handed in (egoFur, files): B itself when
+// it has a compiled transform.  Otherwise (operators built from geometry) a B
+// without a fast transform (2^a x {1, 3, 5}) is embedded into the next fast
+// one -- the circulant only needs L >= S + T - 1, and the lengths with a
+// factor beyond 5 (29, 81, 145) transform several times more slowly.
+gla_status dims_impl(const gla_vol* trg, const gla_vol* src, bool exact, gla_dims* out)
 {
     if (!trg || !out) return fail(GLA_ERR_ARG, "null argument");
     gla_status s = check_vol(*trg, "target volume");
@@ -189,10 +198,10 @@ gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* ou
             return fail(GLA_ERR_SIZE, "the sum of source and target cells must be even on every axis "
                                       "(regenerate the volumes with an even cell count first)");
         d.brn[a] = (d.trg_cel[a] + d.src_cel[a]) / 2;
-        // a branch length without a compiled transform is embedded into the
-        // next compiled one (the circulant only needs L >= S + T - 1); such an
-        // axis stores its full branches
-        d.dev_brn[a] = gla::len_supported((int)d.brn[a]) ? d.brn[a] : gla::len_next_supported((int)d.brn[a]);
+        // an embedded axis stores its full branches
+        const int b = (int)d.brn[a];
+        const bool keep = exact ? gla::len_supported(b) : gla::len_fast(b);
+        d.dev_brn[a] = keep ? d.brn[a] : gla::len_next_supported(b);
         if (d.dev_brn[a] == 0)
             return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) + " exceeds the compiled transforms");
         const bool same_org = trg->org_num[a] * sv.org_den[a] == sv.org_num[a] * trg->org_den[a];
@@ -206,6 +215,15 @@ gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* ou
     }
     *out = d;
     return GLA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* out)
+{
+    return dims_impl(trg, src, true, out);
 }
 
 gla_status gla_opr_get_dims(gla_opr_t h, gla_dims* out)
@@ -458,7 +476,7 @@ gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vo
     gla_status s = gla_init(opts->device);
     if (s) return s;
     gla_dims d;
-    if ((s = gla_opr_dims_for(trg, src, &d))) return s;
+    if ((s = dims_impl(trg, src, false, &d))) return s;
     std::vector<std::complex<double>> fur;
     std::string err;
     s = gla::setup_fur(*opts, *trg, src ? *src : *trg, d, fur, err);
@@ -600,11 +618,18 @@ gla_status gla_opr_load(const char* path, int32_t dtype, int32_t device, gla_opr
     if (hd.dtype != GLA_C64 && hd.dtype != GLA_C128) return bad("unknown element type");
     gla_dims d;
     const gla_vol* sp = hd.self_opr ? nullptr : &hd.src;
-    if (gla_opr_dims_for(&hd.trg, sp, &d) != GLA_OK) return bad("volumes in the header are inconsistent");
-    for (int a = 0; a < 3; ++a)
-        if (d.brn[a] != hd.brn[a] || d.dev_brn[a] != hd.dev_brn[a] || d.fur_ext[a] != hd.fur_ext[a] ||
-            d.sym[a] != hd.sym[a])
-            return bad("branch layout in the header does not match its volumes");
+    // the file holds either the layout of Fourier data handed in or that of
+    // an operator built from geometry
+    bool match = false;
+    for (int exact = 0; exact < 2 && !match; ++exact) {
+        if (dims_impl(&hd.trg, sp, exact != 0, &d) != GLA_OK) return bad("volumes in the header are inconsistent");
+        match = true;
+        for (int a = 0; a < 3; ++a)
+            if (d.brn[a] != hd.brn[a] || d.dev_brn[a] != hd.dev_brn[a] || d.fur_ext[a] != hd.fur_ext[a] ||
+                d.sym[a] != hd.sym[a])
+                match = false;
+    }
+    if (!match) return bad("branch layout in the header does not match its volumes");
     const uint64_t want = 8ull * (uint64_t)(d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2]) * 6ull;
     if (hd.n_elements != want) return bad("element count does not match the branch layout");
     const size_t eb = hd.dtype == GLA_C64 ? 8 : 16;
@@ -1086,10 +1111,19 @@ std::vector<int> zpass_order(const int B[3], const int L[3], const AxisMap am[3]
     return ord;
 }
 
+// the fast lengths, 2^a x {1, 3, 5}, are the embedding targets; the others
+// (29, 81, 145) are compiled for Fourier data that comes in that layout
+bool len_fast(int k)
+{
+    if (k < 1) return false;
+    while (k % 2 == 0) k /= 2;
+    return (k == 1 || k == 3 || k == 5) && len_supported(k == 1 ? 1 : k);
+}
+
 int len_next_supported(int n)
 {
     int best = 0;
-#define GLA_LEN_GE(k) if (k >= n && (best == 0 || k < best)) best = k;
+#define GLA_LEN_GE(k) if (k >= n && len_fast(k) && (best == 0 || k < best)) best = k;
     GLA_FOR_EACH_LEN(GLA_LEN_GE)
 #undef GLA_LEN_GE
     return best;

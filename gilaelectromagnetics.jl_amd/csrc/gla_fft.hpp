@@ -207,20 +207,45 @@ __device__ __forceinline__ void dft16(cpx<R>* a)
     }
 }
 
-// generic odd radix: direct sum with roots taken from the length-N table
+// generic odd radix: direct sum over the pairs (e, RAD - e), whose roots are
+// conjugate: with s_e = a_e + a_{RAD-e}, d_e = a_e - a_{RAD-e} and
+// w^{eq} = C + i S,
+//   out[q] = a_0 + sum_e (s_e C + i d_e S),  out[RAD-q] = a_0 + sum_e (s_e C - i d_e S),
+// so each (q, e) costs two real-by-complex products for two outputs.  The
+// (RAD-1)/2 distinct roots come from the length-N table once per call.
 template <typename R, bool INV, int RAD, int N>
 __device__ __forceinline__ void dftg(cpx<R>* a, const cpx<R>* __restrict__ tw)
 {
+    constexpr int H = (RAD - 1) / 2;
+    static_assert(RAD % 2 == 1, "the generic radix is odd");
+    cpx<R> rt[H + 1];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) rt[k] = tw[k * (N / RAD)];
+    cpx<R> sm[H + 1], df[H + 1];
+    cpx<R> o0 = a[0];
+#pragma unroll
+    for (int e = 1; e <= H; ++e) {
+        sm[e] = a[e] + a[RAD - e];
+        df[e] = a[e] - a[RAD - e];
+        o0 = o0 + sm[e];
+    }
     cpx<R> o[RAD];
+    o[0] = o0;
 #pragma unroll
-    for (int q = 0; q < RAD; ++q) {
-        cpx<R> acc = a[0];
+    for (int q = 1; q <= H; ++q) {
+        cpx<R> X = a[0], Y = {R(0), R(0)};
 #pragma unroll
-        for (int e = 1; e < RAD; ++e) {
-            cpx<R> rt = tw[((e * q) % RAD) * (N / RAD)];
-            acc = acc + (INV ? cmulc(a[e], rt) : a[e] * rt);
+        for (int e = 1; e <= H; ++e) {
+            const int k = (e * q) % RAD;
+            // forward root w^k = e^{-2 pi i k / RAD}; w^k = conj(w^{RAD-k})
+            const R C = k <= H ? rt[k].x : rt[RAD - k].x;
+            const R S = (k <= H ? rt[k].y : -rt[RAD - k].y) * (INV ? R(-1) : R(1));
+            X = X + C * sm[e];
+            Y = Y + S * df[e];
         }
-        o[q] = acc;
+        const cpx<R> iY = {-Y.y, Y.x};
+        o[q] = X + iY;
+        o[RAD - q] = X - iY;
     }
 #pragma unroll
     for (int q = 0; q < RAD; ++q) a[q] = o[q];

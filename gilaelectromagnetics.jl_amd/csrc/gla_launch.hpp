@@ -7,12 +7,15 @@
 #include <vector>
 #include "gla_kernels.hpp"
 
-// Branch lengths B (per-axis FFT lengths) compiled into the device library.
-// Other lengths are reached by embedding into the next listed length (the
-// circulant only needs L >= S + T - 1).
-#define GLA_FOR_EACH_LEN(X)                                                                   \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) \
-        X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512)
+// Branch lengths B (per-axis FFT lengths) compiled into the device library:
+// 2^a 3^b 5^c lengths up to 512, and 29, 81 and 145 (the branch lengths of a
+// 256x34x34 source in a 256x256x128 target, so that Fourier data in the
+// reference's layout for that pair is accepted as it is).  Operators built
+// from geometry reach other lengths by embedding into the next listed length
+// (the circulant only needs L >= S + T - 1).
+#define GLA_FOR_EACH_LEN(X)                                                                          \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(29) X(32) X(40) X(48) X(64) \
+        X(80) X(81) X(96) X(128) X(145) X(160) X(192) X(256) X(320) X(384) X(512)
 
 namespace gla {
 
@@ -52,7 +55,8 @@ struct Epi {
 };
 
 bool len_supported(int n);
-int len_next_supported(int n);   // smallest supported length >= n, or 0
+bool len_fast(int n);             // 2^a x {1, 3, 5}: the lengths operators from geometry embed into
+int len_next_supported(int n);   // smallest fast length >= n, or 0
 
 constexpr int NPASS = 5;
 

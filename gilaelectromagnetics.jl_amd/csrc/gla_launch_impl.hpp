@@ -79,19 +79,29 @@ __host__ __device__ constexpr int plan_exchanges(int N, int E)
 
 // elements per thread: close to `target`, with few LDS exchanges, threads
 // per pencil within `maxtp` and, when `wave` is set, dividing 64 (a pencil
-// inside one wave)
-__host__ __device__ constexpr int pick_e(int N, int target, int maxtp, bool wave = false)
+// inside one wave) -- unless that would leave more than 64 elements per
+// thread (lengths with a large odd factor, e.g. 81, 145), in which case the
+// pencil spans waves (workgroup barriers) with at most 32 elements per thread
+__host__ __device__ constexpr int pick_e_in(int N, int target, int maxtp, bool wave, int emax)
 {
     int best = 0, bestscore = 1 << 30;
     for (int e = 1; e <= N; ++e) {
         if (N % e) continue;
         const int tp = N / e;
-        if (tp > maxtp) continue;
+        if (tp > maxtp || e > emax) continue;
         const int d = e > target ? 2 * (e - target) : target - e;
         const int score = d + 4 * plan_exchanges(N, e) + ((wave && 64 % tp != 0) ? 1000 : 0);
         if (score < bestscore) { bestscore = score; best = e; }
     }
-    return best ? best : N;
+    return best;
+}
+
+__host__ __device__ constexpr int pick_e(int N, int target, int maxtp, bool wave = false)
+{
+    const int e = pick_e_in(N, target, maxtp, wave, N);
+    if (e > 0 && e <= 64) return e;
+    const int f = pick_e_in(N, target, maxtp, false, 32);
+    return f > 0 ? f : (e > 0 ? e : N);
 }
 
 template <typename R>
@@ -133,8 +143,16 @@ struct LenK {
     static constexpr int WL_Z = (GLA_ZWAVE != 0 && 64 % TP_Z == 0 && (TP_Z * PXZ) % 64 == 0) ? 1 : 0;
     static constexpr int PS_Z = WL_Z ? N + (N >> GLA_ZPADSH) + 1 : N;
     // pass 3 runs its three component transforms together when their LDS
-    // buffers fit the budget
-    static constexpr int KB_Z = 3 * PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024 ? 3 : 1;
+    // buffers fit the budget and the plan has no direct odd radix above 16
+    // (a radix-29 butterfly for three components at once would spill)
+    static constexpr int RMAX_Z = [] {
+        const Plan p = make_plan(N, N / TP_Z);
+        int m = 1;
+        for (int i = 0; i < p.nst; ++i) m = p.rad[i] > m ? p.rad[i] : m;
+        return m;
+    }();
+    static constexpr int KB_Z =
+        (RMAX_Z <= 16 && 3 * PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024) ? 3 : 1;
 };
 
 template <typename R, int N>

@@ -41,6 +41,11 @@ constexpr ZShape ztune(int complex_bytes, int n)
             case 384: return {4, 8};
             case 512: return {8, 8};
             // END TUNED C64
+            // lengths with a prime factor 29 (29 threads per column): 8
+            // columns per workgroup keep its 232 threads within one wave per
+            // SIMD, so the radix-29 butterflies get the full register file
+            case 29: return {1, 8};
+            case 145: return {5, 8};
             default: return {8, 16};
         }
 #endif

@@ -77,6 +77,44 @@ def test_external_matvec_parity(gila, orc, gpu, S, T, typ):
     assert orc.rel_l2(y, y_ref) <= TOL[typ]
 
 
+# branch lengths with a factor beyond 5: 29 (prime, one direct radix-29
+# stage), 81 = 3^4 and 145 = 5 x 29, on every axis position (pass 1/5 rows,
+# pass 2/4 bundles, pass 3 columns)
+ODD_SELF = [(29, 81, 4), (145, 4, 29), (4, 145, 81), (6, 29, 145)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", ODD_SELF)
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_self_matvec_odd_lengths(gila, orc, gpu, N, typ):
+    print(f"case self odd lengths N={N} {np.dtype(typ).name}", flush=True)
+    mem, fur = _mem_self(gila, orc, N, typ)
+    assert tuple(mem._dims.dev_brn) == N
+    x = orc.complex_gaussian(N + (3,), seed=21)
+    y_ref = orc.matvec(N, N, (1, 1, 1), fur, x)
+    assert orc.rel_l2(_dev_apply(gila, mem, x), y_ref) <= TOL[typ]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_external_fur_with_branch_lengths_29_and_81(gila, orc, gpu, typ):
+    # Fourier data in the reference's layout for B = (4, 29, 81): accepted as
+    # it is (no re-embedding) and applied to the oracle's bound
+    S, T = (4, 30, 2), (4, 28, 160)
+    print(f"case ext fur B=(4,29,81) {np.dtype(typ).name}", flush=True)
+    crc = orc.crc_ext(S, T, (H, H, H), (0.25, 0.5, -0.75))
+    sym = (0, 0, 0)
+    fur = orc.fur_from_crc(S, T, sym, crc)
+    B, U = orc.dims(S, T, sym)
+    assert B == (4, 29, 81)
+    src = gila.GlaVol(S, ((1, 32),) * 3, (0, 0, 0))
+    trg = gila.GlaVol(T, ((1, 32),) * 3, ((1, 4), (1, 2), (3, 4)))
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), trg, src, egoFur=orc.fur_branches(fur.astype(typ), U), setTyp=typ)
+    assert tuple(mem._dims.dev_brn) == B
+    x = orc.complex_gaussian(S + (3,), seed=22)
+    assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(S, T, sym, fur, x)) <= TOL[typ]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("typ", [np.complex128, np.complex64])
 def test_external_touching_symmetric_axes(gila, orc, gpu, typ):

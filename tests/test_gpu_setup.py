@@ -157,8 +157,10 @@ def test_contained_source_with_reembedded_axes(gila, orc, gpu, typ, tol):
 @pytest.mark.parametrize("typ,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5)])
 def test_guide_shaped_external_full_size(gila, orc, gpu, typ, tol):
     # source 256x34x34 centred in a 256x256x128 target at 1/32 wavelength (the
-    # guide recipe of examples/scattering_guide.jl:102-116): B = (256,145,81),
-    # axes 2 and 3 re-embedded in 160 and 96.  The external operator must equal
+    # guide recipe of examples/scattering_guide.jl:102-116): B = (256,145,81);
+    # built from geometry, axes 2 and 3 run on the next fast transform lengths
+    # 160 and 96 (Fourier data handed in at B itself runs on 145 and 81,
+    # test_external_fur_with_branch_lengths_29_and_81).  The external operator must equal
     # the target's self operator on the embedded source, cropped -- the
     # identity of test/extSlfTest.jl:45-47 at full size.  Input: an x-polarised
     # point dipole in a seeded random source cell plus 1e-3 complex Gaussian noise.
