@@ -242,12 +242,34 @@ def batched_leg(gila, log):
         torch.cuda.synchronize()
         ok = ok and all(torch.equal(o[3], e) for o, e in zip(ops, eager))
     ms_graph = event_ms(g.replay, 10, st)
+    # the library's recorded batch: one graph whose operators are dealt over
+    # four parallel branches (gla_batch_create)
+    n = len(ops)
+    b = ctypes.c_void_p()
+    _capi.check(lib.gla_batch_create((ctypes.c_void_p * n)(*[o[1]._h for o in ops]),
+                                     (ctypes.c_void_p * n)(*[o[2].data_ptr() for o in ops]),
+                                     (ctypes.c_void_p * n)(*[o[3].data_ptr() for o in ops]), None, n, 4,
+                                     ctypes.byref(b)))
+    launch_b = lambda: _capi.check(lib.gla_batch_launch(b, ctypes.c_void_p(st.cuda_stream)))
+    ok_b = True
+    for _ in range(3):
+        for o in ops:
+            o[3].zero_()
+        torch.cuda.synchronize()
+        launch_b()
+        torch.cuda.synchronize()
+        ok_b = ok_b and all(torch.equal(o[3], e) for o, e in zip(ops, eager))
+    ms_batch = event_ms(launch_b, 10, st)
+    _capi.check(lib.gla_batch_destroy(b))
     out["mixed16"] = {"workload": "16 self operators drawn (seeded) from 24^3, 32x48x64, 64^3, 80x40x40, "
                                   "ComplexF32, one matvec each",
                       "shapes": ["x".join(map(str, o[0])) for o in ops],
                       "ms_eager": ms_eager, "ms_hipgraph": ms_graph,
                       "matvecs_per_s_hipgraph": 16 * 1000.0 / ms_graph,
-                      "graph_replays_bitwise_equal_eager": bool(ok)}
+                      "graph_replays_bitwise_equal_eager": bool(ok),
+                      "ms_recorded_batch_4_lanes": ms_batch,
+                      "matvecs_per_s_recorded_batch": 16 * 1000.0 / ms_batch,
+                      "recorded_batch_bitwise_equal_eager": bool(ok_b)}
     del ops, g
     torch.cuda.empty_cache()
     return out

@@ -27,9 +27,10 @@ GLA_C128 = 2
 GLA_NOALLOC_SCRATCH_BYTES = 256 << 20
 
 EXPORTED = (
-    "gla_last_error", "gla_init", "gla_opr_dims_for", "gla_opr_create_from_fur", "gla_opr_create",
+    "gla_last_error", "gla_init", "gla_opr_dims_for", "gla_vol_even_gen", "gla_opr_create_from_fur", "gla_opr_create",
     "gla_opr_export_fur", "gla_opr_get_dims", "gla_opr_adjoint", "gla_opr_destroy", "gla_opr_apply",
-    "gla_opr_apply_axpby", "gla_opr_apply_host", "gla_opr_apply_noalloc", "gla_opr_reserve", "gla_opr_workspace_bytes", "gla_opr_profile",
+    "gla_opr_apply_axpby", "gla_opr_apply_host", "gla_opr_apply_noalloc", "gla_opr_reserve", "gla_batch_create", "gla_batch_launch",
+    "gla_batch_destroy", "gla_opr_workspace_bytes", "gla_opr_profile",
     "gla_opr_profile_seq",
     "gla_ls_apply", "gla_krylov_dots_workspace", "gla_krylov_dots", "gla_krylov_axpy",
     "gla_krylov_rows", "gla_krylov_next", "gla_krylov_givens", "gla_opr_get_vols", "gla_opr_get_opts", "gla_opr_save", "gla_opr_load",
@@ -53,7 +54,8 @@ class gla_dims(ctypes.Structure):
                 ("brn", ctypes.c_int64 * 3), ("fur_ext", ctypes.c_int64 * 3),
                 ("sym", ctypes.c_int32 * 3), ("dtype", ctypes.c_int32),
                 ("adj_mod", ctypes.c_int32), ("self_opr", ctypes.c_int32),
-                ("dev_brn", ctypes.c_int64 * 3)]
+                ("dev_brn", ctypes.c_int64 * 3),
+                ("src_div", ctypes.c_int64 * 3), ("trg_div", ctypes.c_int64 * 3)]
 
 
 class GlaError(RuntimeError):
@@ -77,6 +79,7 @@ def load(path=LIB_PATH):
     sigs = {
         "gla_init": [i32],
         "gla_opr_dims_for": [P(gla_vol), P(gla_vol), P(gla_dims)],
+        "gla_vol_even_gen": [P(gla_vol), P(gla_vol)],
         "gla_opr_create_from_fur": [P(gla_opts), P(gla_vol), P(gla_vol), vp, i32, i64, P(vp)],
         "gla_opr_create": [P(gla_opts), P(gla_vol), P(gla_vol), P(vp)],
         "gla_opr_export_fur": [vp, vp, i64],
@@ -88,6 +91,9 @@ def load(path=LIB_PATH):
         "gla_opr_apply_noalloc": [vp, vp, i64, vp, i64, i64, vp],
         "gla_opr_apply_host": [vp, vp, i64, vp, i64, i64],
         "gla_opr_reserve": [vp, i64],
+        "gla_batch_create": [vp, vp, vp, vp, i32, i32, P(vp)],
+        "gla_batch_launch": [vp, vp],
+        "gla_batch_destroy": [vp],
         "gla_opr_profile": [vp, vp, i64, vp, i64, i64, vp, i32, P(d), P(i64)],
         "gla_opr_profile_seq": [vp, vp, i64, vp, i64, i64, vp, i32, P(d), P(i64), P(d)],
         "gla_ls_apply": [vp, vp, i64, i64, vp, i64, vp, i64, i64, vp],

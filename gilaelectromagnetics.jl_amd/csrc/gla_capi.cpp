@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <complex>
@@ -129,6 +130,7 @@ struct gla_opr_s {
     void* w1 = nullptr;
     void* w2 = nullptr;
     int64_t ws_nrhs = 0;
+    int batches = 0;   // recorded batches that use w1 / w2 (they must not move)
     // device copies of the host arrays of gla_opr_apply_host (grown as
     // needed, kept until release), under their own lock
     void* hx = nullptr;
@@ -166,11 +168,55 @@ gla_status gla_init(int32_t device)
 
 namespace {
 
+// glaVolEveGen (src/glaMem.jl:152-176): odd axes get one more cell of a
+// proportionally smaller size, the extent unchanged
+gla_vol even_gen(const gla_vol& v)
+{
+    gla_vol o = v;
+    for (int a = 0; a < 3; ++a) {
+        if (v.cel[a] % 2 == 0) continue;
+        const int64_t n = v.cel[a] + 1;
+        __int128 num = (__int128)v.scl_num[a] * v.cel[a], den = (__int128)v.scl_den[a] * n;
+        __int128 x = num < 0 ? -num : num, y = den < 0 ? -den : den;
+        while (y) { const __int128 t = x % y; x = y; y = t; }
+        if (x > 1) { num /= x; den /= x; }
+        o.cel[a] = n;
+        o.scl_num[a] = (int64_t)num;
+        o.scl_den[a] = (int64_t)den;
+    }
+    return o;
+}
+
+// an external pair whose cells sum to an odd number on some axis is
+// regenerated on both sides, as GlaOprMem does (src/glaMemSup.jl:34-40)
+void even_pair(const gla_vol* trg, const gla_vol* src, gla_vol& t2, gla_vol& s2)
+{
+    t2 = *trg;
+    s2 = src ? *src : *trg;
+    if (!src || vol_eq(*trg, *src)) return;
+    bool odd = false;
+    for (int a = 0; a < 3; ++a) odd = odd || (trg->cel[a] + src->cel[a]) % 2 != 0;
+    if (odd) {
+        t2 = even_gen(*trg);
+        s2 = even_gen(*src);
+    }
+}
+
+// the integer q with a == q * b for positive rationals a, b, or 0
+int64_t int_ratio(int64_t an, int64_t ad, int64_t bn, int64_t bd)
+{
+    const __int128 num = (__int128)an * bd, den = (__int128)ad * bn;
+    if (den <= 0 || num <= 0 || num % den != 0) return 0;
+    const __int128 q = num / den;
+    return q > (__int128)(1 << 20) ? 0 : (int64_t)q;
+}
+
 // exact: the layout of Fourier data handed in (egoFur, files): B itself when
 // it has a compiled transform.  Otherwise (operators built from geometry) a B
 // without a fast transform (2^a x {1, 3, 5}) is embedded into the next fast
 // one -- the circulant only needs L >= S + T - 1, and the lengths with a
 // factor beyond 5 (29, 81, 145) transform several times more slowly.
+// trg / src are the volumes after regeneration (even_pair).
 gla_status dims_impl(const gla_vol* trg, const gla_vol* src, bool exact, gla_dims* out)
 {
     if (!trg || !out) return fail(GLA_ERR_ARG, "null argument");
@@ -181,22 +227,31 @@ gla_status dims_impl(const gla_vol* trg, const gla_vol* src, bool exact, gla_dim
     if (!self && (s = check_vol(sv, "source volume"))) return s;
     gla_dims d{};
     d.self_opr = self ? 1 : 0;
+    // partitions of GlaExtInf (src/glaMem.jl:190-216): per axis the coarser
+    // scale is the lattice (lcm) and the finer volume splits into
+    // ratio-strided partitions
     for (int a = 0; a < 3; ++a) {
-        // one source and one target partition: equal cell scales
-        if (trg->scl_num[a] * sv.scl_den[a] != sv.scl_num[a] * trg->scl_den[a]) {
-            const long double r = (long double)trg->scl_num[a] * sv.scl_den[a] /
-                                  ((long double)sv.scl_num[a] * trg->scl_den[a]);
-            const bool integral = std::fabs(r - std::round(r)) < 1e-12L || std::fabs(1 / r - std::round(1 / r)) < 1e-12L;
-            if (!integral) return fail(GLA_ERR_SCALE, "Volume pair must share a common scale grid.");
-            return fail(GLA_ERR_UNSUPPORTED,
-                        "source and target cell sizes differ by an integer factor; the partitioned "
-                        "(multi-resolution) operator is not implemented");
+        const int64_t st = int_ratio(trg->scl_num[a], trg->scl_den[a], sv.scl_num[a], sv.scl_den[a]);
+        const int64_t ts = int_ratio(sv.scl_num[a], sv.scl_den[a], trg->scl_num[a], trg->scl_den[a]);
+        if (st == 0 && ts == 0) return fail(GLA_ERR_SCALE, "Volume pair must share a common scale grid.");
+        d.src_div[a] = st > 0 ? st : 1;   // target cells coarser: source partitions
+        d.trg_div[a] = st > 0 ? 1 : ts;
+        if (trg->cel[a] % d.trg_div[a] || sv.cel[a] % d.src_div[a]) {
+            if (trg->cel[a] < d.trg_div[a] || sv.cel[a] < d.src_div[a])
+                return fail(GLA_ERR_SCALE, "Volume sizes are incompatible---volume smaller than cell.");
+            return fail(GLA_ERR_SIZE, "a volume with finer cells must divide into whole partitions: its cell "
+                                      "count on every axis must be a multiple of the scale ratio");
         }
-        d.trg_cel[a] = trg->cel[a];
-        d.src_cel[a] = sv.cel[a];
+    }
+    const int64_t np = d.src_div[0] * d.src_div[1] * d.src_div[2] * d.trg_div[0] * d.trg_div[1] * d.trg_div[2];
+    for (int a = 0; a < 3; ++a) {
+        d.trg_cel[a] = trg->cel[a] / d.trg_div[a];
+        d.src_cel[a] = sv.cel[a] / d.src_div[a];
         if ((d.trg_cel[a] + d.src_cel[a]) % 2)
-            return fail(GLA_ERR_SIZE, "the sum of source and target cells must be even on every axis "
-                                      "(regenerate the volumes with an even cell count first)");
+            return fail(GLA_ERR_SIZE, np > 1 ? "the sum of source and target partition cells must be even on every "
+                                               "axis"
+                                             : "the sum of source and target cells must be even on every axis "
+                                               "(regenerate the volumes with an even cell count first)");
         d.brn[a] = (d.trg_cel[a] + d.src_cel[a]) / 2;
         // an embedded axis stores its full branches
         const int b = (int)d.brn[a];
@@ -205,7 +260,7 @@ gla_status dims_impl(const gla_vol* trg, const gla_vol* src, bool exact, gla_dim
         if (d.dev_brn[a] == 0)
             return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) + " exceeds the compiled transforms");
         const bool same_org = trg->org_num[a] * sv.org_den[a] == sv.org_num[a] * trg->org_den[a];
-        d.sym[a] = (d.trg_cel[a] == d.src_cel[a] && same_org && d.dev_brn[a] == d.brn[a]) ? 1 : 0;
+        d.sym[a] = (d.trg_cel[a] == d.src_cel[a] && same_org && d.dev_brn[a] == d.brn[a] && np == 1) ? 1 : 0;
         if (d.sym[a]) {
             int64_t u = (d.trg_cel[a] + 1) / 2 + (d.trg_cel[a] % 2 == 0);
             d.fur_ext[a] = u < 2 ? 2 : u;
@@ -217,13 +272,35 @@ gla_status dims_impl(const gla_vol* trg, const gla_vol* src, bool exact, gla_dim
     return GLA_OK;
 }
 
+// source and target partition counts
+int64_t n_src_part(const gla_dims& d) { return d.src_div[0] * d.src_div[1] * d.src_div[2]; }
+int64_t n_trg_part(const gla_dims& d) { return d.trg_div[0] * d.trg_div[1] * d.trg_div[2]; }
+
+// complex elements of the Green data in the reference layout
+int64_t fur_count(const gla_dims& d)
+{
+    return 8 * d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2] * 6 * n_src_part(d) * n_trg_part(d);
+}
+
 }  // namespace
 
 extern "C" {
 
 gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* out)
 {
-    return dims_impl(trg, src, true, out);
+    if (!trg || !out) return fail(GLA_ERR_ARG, "null argument");
+    gla_vol t2, s2;
+    even_pair(trg, src, t2, s2);
+    return dims_impl(&t2, src ? &s2 : nullptr, true, out);
+}
+
+gla_status gla_vol_even_gen(const gla_vol* in, gla_vol* out)
+{
+    if (!in || !out) return fail(GLA_ERR_ARG, "null argument");
+    gla_status s = check_vol(*in, "volume");
+    if (s) return s;
+    *out = even_gen(*in);
+    return GLA_OK;
 }
 
 gla_status gla_opr_get_dims(gla_opr_t h, gla_dims* out)
@@ -244,15 +321,19 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
     const gla_dims& d = h->dims;
     const int64_t U0 = d.fur_ext[0], U1 = d.fur_ext[1], U2 = d.fur_ext[2];
     const int64_t Ut = U0 * U1 * U2;
+    const int64_t P = n_src_part(d) * n_trg_part(d);
     const double scale = 1.0 / (double)(8 * d.dev_brn[0] * d.dev_brn[1] * d.dev_brn[2]);
-    // device layout: ((((b*U2 + j2)*U1 + j1)*U0 + j0)*6 + comp
-    std::vector<C> g((size_t)(8 * Ut * 6));
-    for (int b = 0; b < 8; ++b)
-        for (int c = 0; c < 6; ++c)
-            for (int64_t j = 0; j < Ut; ++j) {
-                const std::complex<double> v = furd[(size_t)((b * 6 + c) * Ut + j)] * scale;
-                g[(size_t)(((int64_t)b * Ut + j) * 6 + c)] = C((R)v.real(), (R)v.imag());
-            }
+    // device layout, one block per partition pair p = s + Ps * t:
+    // (((((p*8 + b)*U2 + j2)*U1 + j1)*U0 + j0)*6 + comp; the reference layout
+    // is ((b*P + p)*6 + comp)*Ut + j
+    std::vector<C> g((size_t)(P * 8 * Ut * 6));
+    for (int64_t p = 0; p < P; ++p)
+        for (int b = 0; b < 8; ++b)
+            for (int c = 0; c < 6; ++c)
+                for (int64_t j = 0; j < Ut; ++j) {
+                    const std::complex<double> v = furd[(size_t)(((b * P + p) * 6 + c) * Ut + j)] * scale;
+                    g[(size_t)(((p * 8 + b) * Ut + j) * 6 + c)] = C((R)v.real(), (R)v.imag());
+                }
     // phase and twiddle tables per axis
     int64_t ntab = 0;
     for (int a = 0; a < 3; ++a) ntab += 2 * d.dev_brn[a];
@@ -303,29 +384,88 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
     return GLA_OK;
 }
 
+// launch geometry of partition pair (0, 0); other pairs change only the
+// partition offsets (pair_geom)
 void fill_geom(gla_opr_t h)
 {
     gla::Geom& g = h->geom;
     const gla_dims& d = h->dims;
+    int fs[3], ft[3];
     for (int a = 0; a < 3; ++a) {
         g.S[a] = (int)d.src_cel[a];
         g.T[a] = (int)d.trg_cel[a];
         g.B[a] = (int)d.dev_brn[a];
         g.L[a] = 2 * g.B[a];
+        fs[a] = (int)(d.src_cel[a] * d.src_div[a]);
+        ft[a] = (int)(d.trg_cel[a] * d.trg_div[a]);
     }
     g.Y1 = std::max(g.S[1], g.T[1]);
     g.Z = std::max(g.S[2], g.T[2]);
-    g.xs = (int64_t)g.S[0] * g.S[1] * g.S[2] * 3;
-    g.ys = (int64_t)g.T[0] * g.T[1] * g.T[2] * 3;
+    g.xs = (int64_t)fs[0] * fs[1] * fs[2] * 3;
+    g.ys = (int64_t)ft[0] * ft[1] * ft[2] * 3;
     g.w1s = (int64_t)g.L[0] * g.Y1 * g.Z * 3;
     g.w2s = (int64_t)g.L[0] * g.L[1] * g.Z * 3;
+    g.xp = gla::whole_part(fs);
+    g.yp = gla::whole_part(ft);
+    for (int a = 0; a < 3; ++a) {
+        g.xp.d[a] = (int)d.src_div[a];
+        g.yp.d[a] = (int)d.trg_div[a];
+    }
+}
+
+// partition offsets of partition `p` on a grid of divisors div (x fastest,
+// CartesianIndices order of GlaExtInf's srcPar / trgPar)
+void part_offset(int64_t p, const int64_t div[3], int o[3])
+{
+    o[0] = (int)(p % div[0]);
+    o[1] = (int)((p / div[0]) % div[1]);
+    o[2] = (int)(p / (div[0] * div[1]));
 }
 
 size_t elem_bytes(int dtype) { return dtype == GLA_C64 ? 8 : 16; }
 
+// The whole matvec on `st` with work planes w1 / w2.  A partitioned operator
+// runs the five passes once per (source, target) partition pair: pass 1
+// gathers source partition s (genPrt), pass 5 writes or accumulates into
+// target partition t (mrgPrt); the first source partition of a target
+// partition applies (alpha, beta), the later ones add with beta = 1, which
+// is the leaf sum of egoBrnHst! (src/glaAct.jl:158-164) carried through the
+// inverse transforms (they are linear).
+hipError_t matvec_all(const gla_opr_s& h, const void* x, void* y, void* w1, void* w2, int nrhs, double ar,
+                      double ai, double br, double bi, int axpby, const gla::Epi& epi, hipStream_t st)
+{
+    const bool c64 = h.dims.dtype == GLA_C64;
+    auto one = [&](const gla::Geom& g, const gla::DevTab& tb, double b_r, double b_i, int ax) {
+        return c64 ? gla::run_matvec<float>(g, tb, x, y, w1, w2, nrhs, ar, ai, b_r, b_i, ax, epi, st)
+                   : gla::run_matvec<double>(g, tb, x, y, w1, w2, nrhs, ar, ai, b_r, b_i, ax, epi, st);
+    };
+    const int64_t Ps = n_src_part(h.dims), Pt = n_trg_part(h.dims);
+    if (Ps * Pt == 1) return one(h.geom, h.tab, br, bi, axpby);
+    if (axpby == 2) return hipErrorInvalidValue;   // self operators only, never partitioned
+    if (axpby == 0) {
+        ar = 1; ai = 0; br = 0; bi = 0;
+    }
+    const int64_t blk = 8 * h.dims.fur_ext[0] * h.dims.fur_ext[1] * h.dims.fur_ext[2] * 6 *
+                        (int64_t)elem_bytes(h.dims.dtype);
+    for (int64_t t = 0; t < Pt; ++t)
+        for (int64_t s = 0; s < Ps; ++s) {
+            gla::Geom g = h.geom;
+            part_offset(s, h.dims.src_div, g.xp.o);
+            part_offset(t, h.dims.trg_div, g.yp.o);
+            gla::DevTab tb = h.tab;
+            tb.gf = (const char*)h.tab.gf + (s + Ps * t) * blk;
+            const hipError_t e = one(g, tb, s == 0 ? br : 1.0, s == 0 ? bi : 0.0, 1);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
 gla_status ensure_ws(gla_opr_t h, int64_t nrhs)
 {
     if (h->ws_nrhs >= nrhs) return GLA_OK;
+    if (h->batches > 0)
+        return fail(GLA_ERR_ARG, "the operator's work buffers are recorded in a batch; "
+                                 "destroy the batch before applying it to more right-hand sides");
     const size_t eb = elem_bytes(h->dims.dtype);
     if (h->w1) { (void)hipFree(h->w1); h->w1 = nullptr; }
     if (h->w2) { (void)hipFree(h->w2); h->w2 = nullptr; }
@@ -359,12 +499,7 @@ gla_status run(gla_opr_t h, const void* x, int64_t nx, void* y, int64_t ny, int6
     }
     gla_status s = ensure_ws(h, nrhs);
     if (s) return s;
-    hipError_t e;
-    if (h->dims.dtype == GLA_C64)
-        e = gla::run_matvec<float>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby, epi,
-                                   (hipStream_t)stream);
-    else
-        e = gla::run_matvec<double>(h->geom, h->tab, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby, epi,
+    const hipError_t e = matvec_all(*h, x, y, h->w1, h->w2, (int)nrhs, ar, ai, br, bi, axpby, epi,
                                     (hipStream_t)stream);
     if (e != hipSuccess) return fail(GLA_ERR_HIP, std::string("matvec launch: ") + hipGetErrorString(e));
     return GLA_OK;
@@ -397,14 +532,17 @@ gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, con
     if (fur_dtype != GLA_C64 && fur_dtype != GLA_C128) return fail(GLA_ERR_ARG, "fur dtype must be c64 or c128");
     gla_status s = gla_init(opts->device);
     if (s) return s;
+    gla_vol t2, s2;
+    even_pair(trg, src, t2, s2);
+    const gla_vol* sp = src ? &s2 : nullptr;
     gla_dims d;
-    if ((s = gla_opr_dims_for(trg, src, &d))) return s;
+    if ((s = dims_impl(&t2, sp, true, &d))) return s;
     for (int a = 0; a < 3; ++a)
         if (d.dev_brn[a] != d.brn[a])
             return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) +
                                                  " has no compiled transform; Fourier data in the reference layout "
                                                  "cannot be re-embedded (build the operator with gla_opr_create)");
-    return create_common(opts, trg, src, d, fur, fur_dtype, fur_len, out);
+    return create_common(opts, &t2, sp, d, fur, fur_dtype, fur_len, out);
 }
 
 }  // extern "C"
@@ -416,9 +554,9 @@ gla_status create_common(const gla_opts* opts, const gla_vol* trg, const gla_vol
 {
     gla_status s;
     const int64_t Ut = d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2];
-    if (fur_len != 8 * Ut * 6)
+    if (fur_len != fur_count(d))
         return fail(GLA_ERR_SIZE, "Fourier Green data has " + std::to_string(fur_len) + " elements, expected 8*" +
-                                      std::to_string(Ut) + "*6");
+                                      std::to_string(Ut) + "*6*" + std::to_string(n_src_part(d) * n_trg_part(d)));
     const bool ok = fur_dtype == GLA_C64 ? finite_all((const float*)fur, fur_len)
                                          : finite_all((const double*)fur, fur_len);
     if (!ok) return fail(GLA_ERR_NONFINITE, "Fourier information contains non-numeric values.");
@@ -475,13 +613,16 @@ gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vo
         return fail(GLA_ERR_ARG, "setTyp must be a subtype of Complex (ComplexF32 or ComplexF64)");
     gla_status s = gla_init(opts->device);
     if (s) return s;
+    gla_vol t2, s2;
+    even_pair(trg, src, t2, s2);
+    const gla_vol* sp = src ? &s2 : nullptr;
     gla_dims d;
-    if ((s = dims_impl(trg, src, false, &d))) return s;
+    if ((s = dims_impl(&t2, sp, false, &d))) return s;
     std::vector<std::complex<double>> fur;
     std::string err;
-    s = gla::setup_fur(*opts, *trg, src ? *src : *trg, d, fur, err);
+    s = gla::setup_fur(*opts, t2, sp ? s2 : t2, d, fur, err);
     if (s) return fail(s, err);
-    return create_common(opts, trg, src, d, fur.data(), GLA_C128, (int64_t)fur.size(), out);
+    return create_common(opts, &t2, sp, d, fur.data(), GLA_C128, (int64_t)fur.size(), out);
 }
 
 gla_status gla_opr_export_fur(gla_opr_t h, void* buf, int64_t len)
@@ -630,7 +771,7 @@ gla_status gla_opr_load(const char* path, int32_t dtype, int32_t device, gla_opr
                 match = false;
     }
     if (!match) return bad("branch layout in the header does not match its volumes");
-    const uint64_t want = 8ull * (uint64_t)(d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2]) * 6ull;
+    const uint64_t want = (uint64_t)fur_count(d);
     if (hd.n_elements != want) return bad("element count does not match the branch layout");
     const size_t eb = hd.dtype == GLA_C64 ? 8 : 16;
     std::vector<char> pay((size_t)want * eb);
@@ -711,13 +852,7 @@ gla_status gla_opr_apply_noalloc(gla_opr_t h, const void* x, int64_t nx, void* y
     DevGuard g(h->opts.device);
     void* w1 = base;
     void* w2 = (unsigned char*)base + w1b;
-    hipError_t e;
-    if (h->dims.dtype == GLA_C64)
-        e = gla::run_matvec<float>(h->geom, h->tab, x, y, w1, w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(),
-                                   (hipStream_t)stream);
-    else
-        e = gla::run_matvec<double>(h->geom, h->tab, x, y, w1, w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(),
-                                    (hipStream_t)stream);
+    const hipError_t e = matvec_all(*h, x, y, w1, w2, (int)nrhs, 1, 0, 0, 0, 0, gla::Epi(), (hipStream_t)stream);
     if (e != hipSuccess) return fail(GLA_ERR_HIP, std::string("matvec launch: ") + hipGetErrorString(e));
     return GLA_OK;
 }
@@ -732,6 +867,8 @@ gla_status gla_ls_apply(gla_opr_t h, const void* chi, int64_t nchi, int64_t chi_
                         void* y, int64_t ny, int64_t nrhs, void* stream)
 {
     if (!h || !chi) return fail(GLA_ERR_ARG, "null argument");
+    if (n_src_part(h->dims) * n_trg_part(h->dims) != 1)
+        return fail(GLA_ERR_SIZE, "the Lippmann-Schwinger form needs equal source and target volumes");
     for (int a = 0; a < 3; ++a)
         if (h->geom.S[a] != h->geom.T[a])
             return fail(GLA_ERR_SIZE, "the Lippmann-Schwinger form needs equal source and target volumes");
@@ -792,6 +929,8 @@ gla_status gla_opr_profile(gla_opr_t h, const void* x, int64_t nx, void* y, int6
                            int32_t reps, double* ms, int64_t* bytes)
 {
     if (!h || !ms || !bytes || reps < 1) return fail(GLA_ERR_ARG, "bad profile arguments");
+    if (n_src_part(h->dims) * n_trg_part(h->dims) != 1)
+        return fail(GLA_ERR_UNSUPPORTED, "per-pass profiles cover operators with one partition pair");
     gla_status s = run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);   // validates, allocates, warms up
     if (s) return s;
     s = profile_locked(h, x, y, nrhs, stream, reps, ms, bytes);
@@ -804,6 +943,8 @@ gla_status gla_opr_profile_seq(gla_opr_t h, const void* x, int64_t nx, void* y, 
                                void* stream, int32_t reps, double* ms, int64_t* bytes, double* total_ms)
 {
     if (!h || !ms || !bytes || !total_ms || reps < 1 || reps > 100000) return fail(GLA_ERR_ARG, "bad profile arguments");
+    if (n_src_part(h->dims) * n_trg_part(h->dims) != 1)
+        return fail(GLA_ERR_UNSUPPORTED, "per-pass profiles cover operators with one partition pair");
     gla_status s = run(h, x, nx, y, ny, nrhs, 1, 0, 0, 0, 0, stream);   // validates, allocates, warms up
     if (s) return s;
     std::lock_guard<std::mutex> lk(h->mu);
@@ -905,6 +1046,136 @@ gla_status profile_locked(gla_opr_t h, const void* x, void* y, int64_t nrhs, voi
 }  // namespace
 
 extern "C" {
+
+}  // extern "C"
+
+// A fixed batch of matvecs recorded into one HIP graph: the operators are
+// dealt over `lanes` parallel branches (largest first, each to the lane with
+// the least work so far, an operator listed twice always to the same lane
+// because both share its work buffers), so independent operators too small
+// to fill the GPU run side by side; a replay is one graph launch.
+struct gla_batch_s {
+    int device = 0;
+    std::vector<gla_opr_t> ops;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~gla_batch_s()
+    {
+        DevGuard g(device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (gla_opr_t h : ops) {
+            std::lock_guard<std::mutex> lk(h->mu);
+            --h->batches;
+        }
+    }
+};
+
+extern "C" {
+
+gla_status gla_batch_create(const gla_opr_t* ops, const void* const* x, void* const* y, const int64_t* nrhs,
+                            int32_t n, int32_t lanes, gla_batch_t* out)
+{
+    if (!ops || !x || !y || !out || n < 1) return fail(GLA_ERR_ARG, "empty or null batch");
+    *out = nullptr;
+    if (lanes < 1) lanes = 1;
+    if (lanes > n) lanes = n;
+    const int dev = ops[0] ? ops[0]->opts.device : 0;
+    for (int i = 0; i < n; ++i) {
+        if (!ops[i] || !x[i] || !y[i]) return fail(GLA_ERR_ARG, "null operator or vector in the batch");
+        if (ops[i]->opts.device != dev) return fail(GLA_ERR_ARG, "a batch runs on one device");
+        const int64_t r = nrhs ? nrhs[i] : 1;
+        if (r < 1 || r > 65535) return fail(GLA_ERR_ARG, "nrhs must be in [1, 65535]");
+    }
+    DevGuard g(dev);
+    // work buffers first: no allocation may happen while recording
+    for (int i = 0; i < n; ++i) {
+        std::lock_guard<std::mutex> lk(ops[i]->mu);
+        gla_status s = ensure_ws(ops[i], nrhs ? nrhs[i] : 1);
+        if (s) return s;
+    }
+    // lane assignment by algorithmic bytes
+    std::vector<int> lane((size_t)n, -1), idx((size_t)n);
+    std::vector<double> cost((size_t)n), load((size_t)lanes, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const gla::Geom& G = ops[i]->geom;
+        cost[(size_t)i] = (double)(G.w1s + G.w2s) * (double)(nrhs ? nrhs[i] : 1);
+        idx[(size_t)i] = i;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[(size_t)a] > cost[(size_t)b]; });
+    for (int i : idx) {
+        int l = -1;
+        for (int j = 0; j < n; ++j)
+            if (j != i && ops[j] == ops[i] && lane[(size_t)j] >= 0) l = lane[(size_t)j];
+        if (l < 0) {
+            l = 0;
+            for (int k = 1; k < lanes; ++k)
+                if (load[(size_t)k] < load[(size_t)l]) l = k;
+        }
+        lane[(size_t)i] = l;
+        load[(size_t)l] += cost[(size_t)i];
+    }
+    auto b = std::make_unique<gla_batch_s>();
+    b->device = dev;
+    hipStream_t cap = nullptr;
+    std::vector<hipStream_t> ls((size_t)lanes, nullptr);
+    std::vector<hipEvent_t> ev((size_t)lanes + 1, nullptr);
+    hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    for (int k = 0; k < lanes && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&ls[(size_t)k], hipStreamNonBlocking);
+    for (auto& v : ev)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&v, hipEventDisableTiming);
+    bool capturing = false;
+    if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+        capturing = true;
+        e = hipEventRecord(ev[(size_t)lanes], cap);   // fork
+    }
+    for (int k = 0; k < lanes && e == hipSuccess; ++k) e = hipStreamWaitEvent(ls[(size_t)k], ev[(size_t)lanes], 0);
+    // record in list order within each lane
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        gla_opr_t h = ops[i];
+        std::lock_guard<std::mutex> lk(h->mu);
+        const int r = (int)(nrhs ? nrhs[i] : 1);
+        hipStream_t st = ls[(size_t)lane[(size_t)i]];
+        e = matvec_all(*h, x[i], y[i], h->w1, h->w2, r, 1, 0, 0, 0, 0, gla::Epi(), st);
+    }
+    for (int k = 0; k < lanes && e == hipSuccess; ++k) e = hipEventRecord(ev[(size_t)k], ls[(size_t)k]);
+    for (int k = 0; k < lanes && e == hipSuccess; ++k) e = hipStreamWaitEvent(cap, ev[(size_t)k], 0);   // join
+    if (capturing) {
+        hipGraph_t gr = nullptr;
+        const hipError_t ee = hipStreamEndCapture(cap, &gr);
+        if (e == hipSuccess) e = ee;
+        b->graph = gr;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    for (auto& v : ev)
+        if (v) (void)hipEventDestroy(v);
+    for (auto& st : ls)
+        if (st) (void)hipStreamDestroy(st);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (e != hipSuccess) return fail(GLA_ERR_HIP, std::string("batch recording: ") + hipGetErrorString(e));
+    for (int i = 0; i < n; ++i) {
+        std::lock_guard<std::mutex> lk(ops[i]->mu);
+        ++ops[i]->batches;
+        b->ops.push_back(ops[i]);
+    }
+    *out = b.release();
+    return GLA_OK;
+}
+
+gla_status gla_batch_launch(gla_batch_t b, void* stream)
+{
+    if (!b) return fail(GLA_ERR_ARG, "null batch");
+    DevGuard g(b->device);
+    const hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    return e == hipSuccess ? GLA_OK : fail(GLA_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+}
+
+gla_status gla_batch_destroy(gla_batch_t b)
+{
+    delete b;
+    return GLA_OK;
+}
 
 gla_status gla_opr_reserve(gla_opr_t h, int64_t nrhs)
 {

@@ -44,6 +44,16 @@ template <> struct YK<double> { static constexpr int value = GLA_YK_D; };
 
 namespace gla {
 
+// A partition of a volume as pass 1 reads it (source) or pass 5 writes it
+// (target): cell (i1, i2, i3) of the partition is cell (i_d * d_d + o_d) of
+// the full volume of f_d cells per axis (genPrt / mrgPrt,
+// src/glaActSup.jl:4-18, 939-953).  Unpartitioned: f = the volume, d = 1,
+// o = 0.
+struct Part {
+    int f[3], d[3], o[3];
+};
+
+
 // Copies the length-N root table (tab[0, N)) and split phases (tab[N, 2N))
 // of an axis into LDS for the workgroup's nt threads.
 template <typename R, int N>
@@ -165,7 +175,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
                                                           int S1, int S2, int S3, int Y1, int Z,
                                                           int64_t xs, int64_t ws,
                                                           const cpx<R>* __restrict__ phz,
-                                                          const cpx<R>* __restrict__ tw)
+                                                          const cpx<R>* __restrict__ tw, Part xp)
 {
     constexpr int E = N / TP;
     // one padding slot every 8 positions (stride-8 stage writes hit distinct banks)
@@ -185,15 +195,20 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
     const int row = blockIdx.x * PEN + pen;
     const bool act = row < rows;
     const int i2 = act ? row % S2 : 0, i3 = act ? (row / S2) % S3 : 0, c = act ? row / (S2 * S3) : 0;
-    const cpx<R>* xr = X + (int64_t)blockIdx.y * xs + (int64_t)row * S1;
+    // the row of source partition xp (the whole volume's row without
+    // partitions: xp.f = S, xp.d = 1, xp.o = 0)
+    const cpx<R>* xr = X + (int64_t)blockIdx.y * xs +
+                       ((((int64_t)c * xp.f[2] + (int64_t)i3 * xp.d[2] + xp.o[2]) * xp.f[1] + (int64_t)i2 * xp.d[1] +
+                         xp.o[1]) * xp.f[0] + xp.o[0]);
+    const int xd = xp.d[0];
     cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
     cpx<R> v[2][E];   // even, odd parity
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int n = t + TP * e;
         cpx<R> a = {R(0), R(0)}, b = {R(0), R(0)};
-        if (act && n < S1) a = xr[n];
-        if (act && n + N < S1) b = xr[n + N];
+        if (act && n < S1) a = xr[(int64_t)n * xd];
+        if (act && n + N < S1) b = xr[(int64_t)(n + N) * xd];
         v[0][e] = a + b;
         v[1][e] = phz[n] * (a - b);
     }
@@ -224,7 +239,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
                                                           const cpx<R>* __restrict__ tw,
                                                           cpx<R> alpha, cpx<R> beta, int axpby,
                                                           const cpx<R>* __restrict__ lsx, int64_t lsxs,
-                                                          const cpx<R>* __restrict__ chi, int64_t chis)
+                                                          const cpx<R>* __restrict__ chi, int64_t chis, Part yp)
 {
     // axpby 0: y = G x;  1: y = alpha G x + beta y;  2: y = x - chi (G x),
     // chi one value per target cell (per right-hand side when chis > 0)
@@ -245,7 +260,13 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     const bool act = row < rows;
     const int i2 = act ? row % T2 : 0, i3 = act ? (row / T2) % T3 : 0, c = act ? row / (T2 * T3) : 0;
     const cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
-    cpx<R>* yr = Y + (int64_t)blockIdx.y * ys + (int64_t)row * T1;
+    // the row of target partition yp (the whole volume's row without
+    // partitions); the Lippmann-Schwinger form (axpby 2) is for self
+    // operators, never partitioned
+    cpx<R>* yr = Y + (int64_t)blockIdx.y * ys +
+                 ((((int64_t)c * yp.f[2] + (int64_t)i3 * yp.d[2] + yp.o[2]) * yp.f[1] + (int64_t)i2 * yp.d[1] + yp.o[1]) *
+                      yp.f[0] + yp.o[0]);
+    const int yd = yp.d[0];
     const int64_t cell0 = ((int64_t)i3 * T2 + i2) * T1;
     cpx<R> v[2][E];
 #pragma unroll
@@ -267,9 +288,10 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     const cpx<R>* xr = axpby == 2 ? lsx + (int64_t)blockIdx.y * lsxs + (int64_t)row * T1 : nullptr;
     const cpx<R>* cr = axpby == 2 ? chi + (int64_t)blockIdx.y * chis + cell0 : nullptr;
     auto epilogue = [&](cpx<R> r, int n) {
-        if (axpby == 1) r = alpha * r + (rd ? beta * yr[n] : cpx<R>{R(0), R(0)});
+        const int64_t ny = (int64_t)n * yd;
+        if (axpby == 1) r = alpha * r + (rd ? beta * yr[ny] : cpx<R>{R(0), R(0)});
         else if (axpby == 2) r = xr[n] - cr[n] * r;
-        yr[n] = r;
+        yr[ny] = r;
     };
 #pragma unroll
     for (int e = 0; e < E; ++e) {

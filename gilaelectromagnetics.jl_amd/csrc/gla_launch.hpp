@@ -20,11 +20,20 @@
 namespace gla {
 
 struct Geom {
-    int S[3], T[3], B[3], L[3];
+    int S[3], T[3], B[3], L[3];   // partition cells (the whole volume without partitions)
     int Y1, Z;             // padded workspace extents: max(S2,T2), max(S3,T3)
-    int64_t xs, ys;        // per-RHS strides of X and Y (elements)
+    int64_t xs, ys;        // per-RHS strides of X and Y (elements, whole volumes)
     int64_t w1s, w2s;      // per-RHS strides of W1 and W2 (elements)
+    Part xp, yp;           // source / target partition being applied
 };
+
+// the whole volume of n[0] x n[1] x n[2] cells as its only partition
+inline Part whole_part(const int* n)
+{
+    Part p{};
+    for (int a = 0; a < 3; ++a) { p.f[a] = n[a]; p.d[a] = 1; p.o[a] = 0; }
+    return p;
+}
 
 struct DevTab {
     const void* phz[3];    // phi_d[n] = e^{-i pi n / B_d}, n < B_d

@@ -163,7 +163,7 @@ static hipError_t launch_fwd_x(const Geom& g, const DevTab& tb, const void* X, v
     dim3 grid((rows + K::PEN - 1) / K::PEN, nrhs);
     hipLaunchKernelGGL((k_fwd_x<R, N, K::TP_ROW, K::PEN, 1>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
                        (const cpx<R>*)X, (cpx<R>*)W1, g.S[0], g.S[1], g.S[2], g.Y1, g.Z, g.xs, g.w1s,
-                       (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0]);
+                       (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0], g.xp);
     return hipGetLastError();
 }
 
@@ -177,7 +177,7 @@ static hipError_t launch_inv_x(const Geom& g, const DevTab& tb, const void* W1, 
     hipLaunchKernelGGL((k_inv_x<R, N, K::TP_ROW, K::PEN, 1>), grid, dim3(K::TP_ROW * K::PEN), 0, st,
                        (const cpx<R>*)W1, (cpx<R>*)Y, g.T[0], g.T[1], g.T[2], g.Y1, g.Z, g.w1s, g.ys,
                        (const cpx<R>*)tb.phz[0], (const cpx<R>*)tb.tw[0], al, be, axpby,
-                       (const cpx<R>*)epi.x, epi.xs, (const cpx<R>*)epi.chi, epi.chis);
+                       (const cpx<R>*)epi.x, epi.xs, (const cpx<R>*)epi.chi, epi.chis, g.yp);
     return hipGetLastError();
 }
 

@@ -34,7 +34,8 @@ namespace {
 constexpr int QMAX = 24;   // largest Gauss-Legendre order in the table
 
 struct GreenCfg {
-    double scl[3];          // cell size per axis (wavelengths)
+    double scl[3];          // cell size per axis (wavelengths); target cells when they differ
+    double sclS[3];         // source cell size (equal to scl except for partitioned operators)
     double zr, zi;          // frqPhz
     double kr, ki;          // 2 pi frqPhz
     double cr, ci;          // 1 / (4 pi frqPhz^2)
@@ -415,6 +416,177 @@ __global__ void k_embed_self(double2* __restrict__ crc, const double2* __restric
     for (int k = 0; k < 6; ++k) crc[k * Lt + p] = make_double2(sg[k] * s[k].x, sg[k] * s[k].y);
 }
 
+// ---- partitioned operators: target and source cells of different sizes ----
+// (the volumes never touch: every face pair is regular)
+
+// overlap length at difference w of an interval of length ha and one of
+// length hb whose centres differ by c: a trapezoid on [c - (ha+hb)/2,
+// c + (ha+hb)/2] with a flat top min(ha, hb) over |w - c| <= |ha - hb| / 2
+__device__ __forceinline__ double trap(double w, double c, double ha, double hb)
+{
+    return fmax(0.0, fmin(fmin(ha, hb), 0.5 * (ha + hb) - fabs(w - c)));
+}
+
+// sorted distinct breakpoints of a trapezoid's support (ends, both kinks and
+// the origin when inside, where 1/r is steepest); returns their count
+__device__ int trap_breaks(double c, double ha, double hb, double* b)
+{
+    const double hw = 0.5 * (ha + hb), kw = 0.5 * fabs(ha - hb), h = fmax(ha, hb);
+    double v[5] = {c - hw, c - kw, c + kw, c + hw, 0.0};
+    const int n = (c - hw < 0.0 && c + hw > 0.0) ? 5 : 4;
+    for (int k = 0; k < n; ++k)
+        if (fabs(v[k]) < SNAP * h) v[k] = 0.0;
+    for (int k = 1; k < n; ++k)
+        for (int j = k; j > 0 && v[j] < v[j - 1]; --j) {
+            const double t = v[j];
+            v[j] = v[j - 1];
+            v[j - 1] = t;
+        }
+    int m = 0;
+    for (int k = 0; k < n; ++k)
+        if (m == 0 || v[k] > b[m - 1] + SNAP * h) b[m++] = v[k];
+    return m;
+}
+
+// breakpoints of a plain interval [lo, hi] split at the origin when inside
+__device__ int span_breaks(double lo, double hi, double* b)
+{
+    int m = 0;
+    b[m++] = lo;
+    if (lo < 0.0 && hi > 0.0) b[m++] = 0.0;
+    b[m++] = hi;
+    return m;
+}
+
+// parallel faces at normal distance d: the integral over (wa, wb) of
+// g(sqrt(wa^2 + wb^2 + d^2)) trap_a(wa) trap_b(wb)
+__device__ Cplx par_mixed(const GreenCfg& G, double d, double ca, double ta, double sa, double cb, double tb,
+                          double sb, int q, const double* gl)
+{
+    double ba[6], bb[6];
+    const int na = trap_breaks(ca, ta, sa, ba), nb = trap_breaks(cb, tb, sb, bb);
+    const double* x = gl_tab(gl, q);
+    double ar = 0, ai = 0;
+    for (int i = 0; i + 1 < na; ++i)
+        for (int j = 0; j + 1 < nb; ++j) {
+            const double a0 = ba[i], a1 = ba[i + 1], b0 = bb[j], b1 = bb[j + 1];
+            for (int k = 0; k < q; ++k) {
+                const double wa = a0 + (a1 - a0) * x[2 * k], fa = (a1 - a0) * x[2 * k + 1] * trap(wa, ca, ta, sa);
+                for (int l = 0; l < q; ++l) {
+                    const double wb = b0 + (b1 - b0) * x[2 * l];
+                    const double r = sqrt(wa * wa + wb * wb + d * d);
+                    const Cplx g = rg(G, r);
+                    const double f = fa * (b1 - b0) * x[2 * l + 1] * trap(wb, cb, tb, sb) / r;
+                    ar += f * g.r;
+                    ai += f * g.i;
+                }
+            }
+        }
+    return Cplx{ar, ai};
+}
+
+// perpendicular faces: the integral of trap_c(w) g(sqrt(w^2 + u^2 + v^2))
+// over w (shared axis), u in [u0, u1] and v in [v0, v1]
+__device__ Cplx perp_mixed(const GreenCfg& G, double cc, double tc, double sc, double u0, double u1, double v0,
+                           double v1, int q, const double* gl)
+{
+    double bw[6], bu[3], bv[3];
+    const int nw = trap_breaks(cc, tc, sc, bw), nu = span_breaks(u0, u1, bu), nv = span_breaks(v0, v1, bv);
+    const double* x = gl_tab(gl, q);
+    double ar = 0, ai = 0;
+    for (int i = 0; i + 1 < nw; ++i)
+        for (int j = 0; j + 1 < nu; ++j)
+            for (int k = 0; k + 1 < nv; ++k) {
+                const double w0 = bw[i], w1 = bw[i + 1], a0 = bu[j], a1 = bu[j + 1], b0 = bv[k], b1 = bv[k + 1];
+                for (int e = 0; e < q; ++e) {
+                    const double w = w0 + (w1 - w0) * x[2 * e], fw = (w1 - w0) * x[2 * e + 1] * trap(w, cc, tc, sc);
+                    for (int l = 0; l < q; ++l) {
+                        const double u = a0 + (a1 - a0) * x[2 * l], fu = fw * (a1 - a0) * x[2 * l + 1];
+                        for (int m = 0; m < q; ++m) {
+                            const double v = b0 + (b1 - b0) * x[2 * m];
+                            const double r = sqrt(w * w + u * u + v * v);
+                            const Cplx g = rg(G, r);
+                            const double f = fu * (b1 - b0) * x[2 * m + 1] / r;
+                            ar += f * g.r;
+                            ai += f * g.i;
+                        }
+                    }
+                }
+            }
+    return Cplx{ar, ai};
+}
+
+// the six components between a target cell (sizes G.scl) and a source cell
+// (G.sclS) at separation sep, by the srfSum! combination as green_at forms
+// it.  Each face pair carries the reference's srfScl factor
+// (src/glaInt.jl:284-305) over the two face areas that its unit-square
+// parametrisation divides out: srfScl indexes its table with the target
+// face where the source face belongs and back, so the factor is
+// area_S(normal of ft) / (A_T(ft) A_S(fs) hT[normal of fs]), with area_S a
+// face area in source-cell sizes -- 1 / V_T for cells of equal shape.
+__device__ void green_mixed(const GreenCfg& G, const double sep[3], const double* __restrict__ gl, double out[12])
+{
+    const double* hT = G.scl;
+    const double* hS = G.sclS;
+    double D = 0;
+    for (int d = 0; d < 3; ++d) D = fmax(D, fabs(sep[d]) / (0.5 * (hT[d] + hS[d])));
+    const int q = reg_order(G, D);
+    double A[3][3][2];
+    for (int nt = 0; nt < 3; ++nt)
+        for (int ns = nt; ns < 3; ++ns) {
+            const int at = (nt + 1) % 3, bt = (nt + 2) % 3, as = (ns + 1) % 3, bs = (ns + 2) % 3;
+            const double fac = (hS[at] * hS[bt]) / ((hT[at] * hT[bt]) * (hS[as] * hS[bs]) * hT[ns]);
+            double acc_r = 0, acc_i = 0;
+            for (int st = 0; st < 2; ++st)
+                for (int ss = 0; ss < 2; ++ss) {
+                    Cplx v;
+                    if (nt == ns) {
+                        const double d = sep[nt] + (st - 0.5) * hT[nt] - (ss - 0.5) * hS[nt];
+                        v = par_mixed(G, d, sep[at], hT[at], hS[at], sep[bt], hT[bt], hS[bt], q, gl);
+                    } else {
+                        const int c = 3 - nt - ns;
+                        const double ut = sep[nt] + (st - 0.5) * hT[nt], vs = sep[ns] - (ss - 0.5) * hS[ns];
+                        v = perp_mixed(G, sep[c], hT[c], hS[c], ut - 0.5 * hS[nt], ut + 0.5 * hS[nt],
+                                       vs - 0.5 * hT[ns], vs + 0.5 * hT[ns], q, gl);
+                    }
+                    const double sg = (st == ss ? 1.0 : -1.0) * (nt == ns ? 1.0 : -1.0);
+                    acc_r += sg * v.r;
+                    acc_i += sg * v.i;
+                }
+            A[nt][ns][0] = acc_r * fac;
+            A[nt][ns][1] = acc_i * fac;
+        }
+    out[0] = A[1][1][0] + A[2][2][0];  out[1] = A[1][1][1] + A[2][2][1];
+    out[2] = A[0][0][0] + A[2][2][0];  out[3] = A[0][0][1] + A[2][2][1];
+    out[4] = A[0][0][0] + A[1][1][0];  out[5] = A[0][0][1] + A[1][1][1];
+    out[6] = A[0][1][0];  out[7] = A[0][1][1];
+    out[8] = A[0][2][0];  out[9] = A[0][2][1];
+    out[10] = A[1][2][0]; out[11] = A[1][2][1];
+}
+
+// circulant of one (source, target) partition pair: separations on the
+// lattice of step M (the coarser cell size per axis, the grid step of both
+// partitions, src/glaMemSup.jl:64-89 with sepGrd, src/glaGen.jl:603-629);
+// zero for T <= n <= L - S as k_crc_ext
+__global__ void k_crc_prt(double2* __restrict__ crc, int T1, int T2, int T3, int S1, int S2, int S3, int L1, int L2,
+                          int L3, double d1, double d2, double d3, double m1, double m2, double m3, GreenCfg G,
+                          const double* __restrict__ gl)
+{
+    const int64_t Lt = (int64_t)L1 * L2 * L3;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Lt) return;
+    const int n1 = (int)(p % L1), n2 = (int)((p / L1) % L2), n3 = (int)(p / ((int64_t)L1 * L2));
+    if ((n1 >= T1 && n1 <= L1 - S1) || (n2 >= T2 && n2 <= L2 - S2) || (n3 >= T3 && n3 <= L3 - S3)) {
+        for (int k = 0; k < 6; ++k) crc[k * Lt + p] = make_double2(0, 0);
+        return;
+    }
+    const double sep[3] = {d1 + (n1 < T1 ? n1 : n1 - L1) * m1, d2 + (n2 < T2 ? n2 : n2 - L2) * m2,
+                           d3 + (n3 < T3 ? n3 : n3 - L3) * m3};
+    double o[12];
+    green_mixed(G, sep, gl, o);
+    for (int k = 0; k < 6; ++k) crc[k * Lt + p] = make_double2(o[2 * k], o[2 * k + 1]);
+}
+
 // external circulant over target-minus-source separations
 // (zero for T <= n <= L - S on an axis; L = S + T is the reference's grid)
 __global__ void k_crc_ext(double2* __restrict__ crc, int T1, int T2, int T3, int S1, int S2, int S3, int L1, int L2,
@@ -436,11 +608,12 @@ __global__ void k_crc_ext(double2* __restrict__ crc, int T1, int T2, int T3, int
     for (int k = 0; k < 6; ++k) crc[k * Lt + p] = make_double2(o[2 * k], o[2 * k + 1]);
 }
 
-// branch extraction with truncation: fur[((b*6 + c)*U3 + j3)*U2 + j2)*U1 + j1]
-// from the split-transformed grid W (L1,L2,L3, comp6), parity-major per axis;
-// a branch shorter than its stored extent repeats its last entry
+// branch extraction with truncation: fur[((((b*P + pair)*6 + c)*U3 + j3)*U2 +
+// j2)*U1 + j1] (partition pair `pair` of P) from the split-transformed grid W
+// (L1,L2,L3, comp6), parity-major per axis; a branch shorter than its stored
+// extent repeats its last entry
 __global__ void k_gather(double2* __restrict__ fur, const double2* __restrict__ W, int B1, int B2, int B3, int U1,
-                         int U2, int U3)
+                         int U2, int U3, int pair, int P)
 {
     const int64_t Ut = (int64_t)U1 * U2 * U3;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -451,7 +624,7 @@ __global__ void k_gather(double2* __restrict__ fur, const double2* __restrict__ 
     const int k1 = j1 < B1 ? j1 : B1 - 1, k2 = j2 < B2 ? j2 : B2 - 1, k3 = j3 < B3 ? j3 : B3 - 1;
     const int m1 = ((b >> 2) & 1) * B1 + k1, m2 = ((b >> 1) & 1) * B2 + k2, m3 = (b & 1) * B3 + k3;
     const int64_t L1 = 2 * B1, L2 = 2 * B2, Lt = L1 * L2 * 2 * B3;
-    fur[p] = W[c * Lt + m1 + L1 * (m2 + L2 * (int64_t)m3)];
+    fur[(((int64_t)b * P + pair) * 6 + c) * Ut + j] = W[c * Lt + m1 + L1 * (m2 + L2 * (int64_t)m3)];
 }
 
 __global__ void k_count_nonfinite(const double2* __restrict__ v, int64_t n, int* __restrict__ cnt)
@@ -507,7 +680,13 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
             return GLA_ERR_UNSUPPORTED;
         }
     GreenCfg G{};
-    for (int a = 0; a < 3; ++a) G.scl[a] = rat(trg.scl_num[a], trg.scl_den[a]);
+    for (int a = 0; a < 3; ++a) {
+        G.scl[a] = rat(trg.scl_num[a], trg.scl_den[a]);
+        G.sclS[a] = rat(src.scl_num[a], src.scl_den[a]);
+    }
+    const int Ps = (int)(d.src_div[0] * d.src_div[1] * d.src_div[2]);
+    const int Pt = (int)(d.trg_div[0] * d.trg_div[1] * d.trg_div[2]);
+    const bool parts = Ps * Pt > 1;
     G.zr = opts.frq_phz_re;
     G.zi = opts.frq_phz_im;
     if (G.zr == 0 && G.zi == 0) {
@@ -522,7 +701,8 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
         G.ci = c.imag();
         // a regular rule integrates the phase factor across a cell to about
         // 1e-15 from 4 + 3 k h nodes per axis
-        const double hmax = std::max(G.scl[0], std::max(G.scl[1], G.scl[2]));
+        double hmax = 0;
+        for (int a = 0; a < 3; ++a) hmax = std::max(hmax, std::max(G.scl[a], G.sclS[a]));
         G.qphase = std::min(QMAX, (int)std::ceil(4.0 + 3.0 * 2.0 * M_PI * std::abs(z) * hmax));
     }
     if (!d.self_opr) {
@@ -530,24 +710,31 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
         // volume along every axis
         double sl[3], su[3], tl[3], tu[3];
         for (int a = 0; a < 3; ++a) {
-            const double h = G.scl[a];
             const double so = rat(src.org_num[a], src.org_den[a]), to = rat(trg.org_num[a], trg.org_den[a]);
-            sl[a] = so - 0.5 * h * src.cel[a];
-            su[a] = so + 0.5 * h * src.cel[a];
-            tl[a] = to - 0.5 * h * trg.cel[a];
-            tu[a] = to + 0.5 * h * trg.cel[a];
+            sl[a] = so - 0.5 * G.sclS[a] * src.cel[a];
+            su[a] = so + 0.5 * G.sclS[a] * src.cel[a];
+            tl[a] = to - 0.5 * G.scl[a] * trg.cel[a];
+            tu[a] = to + 0.5 * G.scl[a] * trg.cel[a];
         }
-        bool overlap = true, aligned = true;
+        bool overlap = true, aligned = true, touch = true;
         for (int a = 0; a < 3; ++a) {
             if (!(tl[a] < su[a] - 1e-12 && sl[a] < tu[a] - 1e-12)) overlap = false;
+            if (!(tl[a] <= su[a] + 1e-12 && sl[a] <= tu[a] + 1e-12)) touch = false;
             const double off = (tl[a] - sl[a]) / G.scl[a];
             if (std::fabs(off - std::round(off)) > 1e-9) aligned = false;
         }
         // volumes whose cell grids coincide may overlap: the operator is then
         // the self operator of the covering grid restricted to the two volumes
-        if (overlap && !aligned) {
+        if (overlap && (!aligned || parts)) {
             err = "Source and target volumes are overlapping.";
             return GLA_ERR_OVERLAP;
+        }
+        // cells of different sizes in contact need the contact-volume
+        // construction of egoFunExtCnt! (src/glaGen.jl:192-230), not built
+        if (touch && parts) {
+            err = "source and target volumes with different cell sizes touch; partitioned operators are built "
+                  "for separated volumes only";
+            return GLA_ERR_UNSUPPORTED;
         }
     }
     std::vector<double> gl((size_t)(QMAX * (QMAX + 1)));
@@ -563,9 +750,10 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
     const int B[3] = {(int)d.dev_brn[0], (int)d.dev_brn[1], (int)d.dev_brn[2]};
     const int L[3] = {2 * B[0], 2 * B[1], 2 * B[2]};
     const int64_t Lt = (int64_t)L[0] * L[1] * L[2];
-    // singular face-pair values of touching cells (DIRECTFN order intOrd)
-    std::complex<double> w18[18];
-    {
+    // singular face-pair values of touching cells (DIRECTFN order intOrd);
+    // partitioned volumes never touch
+    std::complex<double> w18[18] = {};
+    if (!parts) {
         const gla_status st = singular_values(G.scl, G.zr, G.zi, opts.int_ord > 0 ? opts.int_ord : 32, w18, err);
         if (st) return st;
     }
@@ -575,33 +763,6 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
     SETUP_HIP(hipMalloc(&dw18.p, sizeof(w18)));
     SETUP_HIP(hipMemcpy(dw18.p, w18, sizeof(w18), hipMemcpyHostToDevice));
     SETUP_HIP(hipMalloc(&dcrc.p, (size_t)Lt * 6 * sizeof(double2)));
-    const int thr = 128;
-    if (d.self_opr) {
-        const int64_t Nt = d.trg_cel[0] * d.trg_cel[1] * d.trg_cel[2];
-        SETUP_HIP(hipMalloc(&dtoe.p, (size_t)Nt * 6 * sizeof(double2)));
-        hipLaunchKernelGGL(k_toeplitz, dim3((unsigned)((Nt + thr - 1) / thr)), dim3(thr), 0, nullptr, (double2*)dtoe.p,
-                           (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], G, (const double*)dgl.p,
-                           (const double2*)dw18.p);
-        SETUP_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_embed_self, dim3((unsigned)((Lt + 255) / 256)), dim3(256), 0, nullptr, (double2*)dcrc.p,
-                           (const double2*)dtoe.p, (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], L[0], L[1],
-                           L[2]);
-        SETUP_HIP(hipGetLastError());
-    } else {
-        // separation of the first target cell centre from the first source one
-        double del[3];
-        for (int a = 0; a < 3; ++a) {
-            const double h = G.scl[a];
-            const double t0 = rat(trg.org_num[a], trg.org_den[a]) - h * (trg.cel[a] / 2 - (trg.cel[a] % 2 == 0 ? 0.5 : 0));
-            const double s0 = rat(src.org_num[a], src.org_den[a]) - h * (src.cel[a] / 2 - (src.cel[a] % 2 == 0 ? 0.5 : 0));
-            del[a] = t0 - s0;
-        }
-        hipLaunchKernelGGL(k_crc_ext, dim3((unsigned)((Lt + thr - 1) / thr)), dim3(thr), 0, nullptr, (double2*)dcrc.p,
-                           (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], (int)d.src_cel[0],
-                           (int)d.src_cel[1], (int)d.src_cel[2], L[0], L[1], L[2], del[0], del[1], del[2], G,
-                           (const double*)dgl.p, (const double2*)dw18.p);
-        SETUP_HIP(hipGetLastError());
-    }
     // forward transform over the full L grid: S = L on every axis
     Geom g{};
     for (int a = 0; a < 3; ++a) { g.S[a] = L[a]; g.T[a] = L[a]; g.B[a] = B[a]; g.L[a] = L[a]; }
@@ -611,6 +772,8 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
     g.ys = Lt * 3;
     g.w1s = Lt * 3;
     g.w2s = Lt * 3;
+    g.xp = whole_part(g.S);
+    g.yp = whole_part(g.T);
     std::vector<double2> tab;
     int64_t off[3][2];
     for (int a = 0; a < 3; ++a) {
@@ -629,25 +792,75 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
     }
     SETUP_HIP(hipMalloc(&dw1.p, (size_t)Lt * 6 * sizeof(double2)));
     SETUP_HIP(hipMalloc(&dw2.p, (size_t)Lt * 6 * sizeof(double2)));
-    SETUP_HIP(run_fwd3<double>(g, tb, dcrc.p, dw1.p, dw2.p, 2, nullptr));
     const int64_t Ut = d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2];
-    SETUP_HIP(hipMalloc(&dfur.p, (size_t)Ut * 48 * sizeof(double2)));
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)((Ut * 48 + 255) / 256)), dim3(256), 0, nullptr, (double2*)dfur.p,
-                       (const double2*)dw2.p, B[0], B[1], B[2], (int)d.fur_ext[0], (int)d.fur_ext[1],
-                       (int)d.fur_ext[2]);
-    SETUP_HIP(hipGetLastError());
+    const int P = Ps * Pt;
+    SETUP_HIP(hipMalloc(&dfur.p, (size_t)Ut * 48 * P * sizeof(double2)));
     SETUP_HIP(hipMalloc(&dcnt.p, sizeof(int)));
     SETUP_HIP(hipMemset(dcnt.p, 0, sizeof(int)));
-    hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)((Lt * 6 + 255) / 256)), dim3(256), 0, nullptr,
-                       (const double2*)dcrc.p, Lt * 6, (int*)dcnt.p);
-    SETUP_HIP(hipGetLastError());
+    const int thr = 128;
+    // centre of the first cell of each whole volume
+    double t0[3], s0[3];
+    for (int a = 0; a < 3; ++a) {
+        t0[a] = rat(trg.org_num[a], trg.org_den[a]) - G.scl[a] * (trg.cel[a] / 2 - (trg.cel[a] % 2 == 0 ? 0.5 : 0));
+        s0[a] = rat(src.org_num[a], src.org_den[a]) - G.sclS[a] * (src.cel[a] / 2 - (src.cel[a] % 2 == 0 ? 0.5 : 0));
+    }
+    for (int pair = 0; pair < P; ++pair) {
+        if (d.self_opr) {
+            const int64_t Nt = d.trg_cel[0] * d.trg_cel[1] * d.trg_cel[2];
+            SETUP_HIP(hipMalloc(&dtoe.p, (size_t)Nt * 6 * sizeof(double2)));
+            hipLaunchKernelGGL(k_toeplitz, dim3((unsigned)((Nt + thr - 1) / thr)), dim3(thr), 0, nullptr,
+                               (double2*)dtoe.p, (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2], G,
+                               (const double*)dgl.p, (const double2*)dw18.p);
+            SETUP_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_embed_self, dim3((unsigned)((Lt + 255) / 256)), dim3(256), 0, nullptr,
+                               (double2*)dcrc.p, (const double2*)dtoe.p, (int)d.trg_cel[0], (int)d.trg_cel[1],
+                               (int)d.trg_cel[2], L[0], L[1], L[2]);
+            SETUP_HIP(hipGetLastError());
+        } else if (!parts) {
+            // separation of the first target cell centre from the first source one
+            hipLaunchKernelGGL(k_crc_ext, dim3((unsigned)((Lt + thr - 1) / thr)), dim3(thr), 0, nullptr,
+                               (double2*)dcrc.p, (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2],
+                               (int)d.src_cel[0], (int)d.src_cel[1], (int)d.src_cel[2], L[0], L[1], L[2],
+                               t0[0] - s0[0], t0[1] - s0[1], t0[2] - s0[2], G, (const double*)dgl.p,
+                               (const double2*)dw18.p);
+            SETUP_HIP(hipGetLastError());
+        } else {
+            // pair = s + Ps t: partition s of the source (offsets os, x
+            // fastest) and t of the target; cell i of a partition is cell
+            // i div + o of its volume, so the pair's first separation is
+            // t0 + ot hT - s0 - os hS and its lattice step the coarser size
+            const int s = pair % Ps, t = pair / Ps;
+            const int os[3] = {(int)(s % d.src_div[0]), (int)((s / d.src_div[0]) % d.src_div[1]),
+                               (int)(s / (d.src_div[0] * d.src_div[1]))};
+            const int ot[3] = {(int)(t % d.trg_div[0]), (int)((t / d.trg_div[0]) % d.trg_div[1]),
+                               (int)(t / (d.trg_div[0] * d.trg_div[1]))};
+            double del[3], M[3];
+            for (int a = 0; a < 3; ++a) {
+                del[a] = (t0[a] + ot[a] * G.scl[a]) - (s0[a] + os[a] * G.sclS[a]);
+                M[a] = std::max(G.scl[a], G.sclS[a]);
+            }
+            hipLaunchKernelGGL(k_crc_prt, dim3((unsigned)((Lt + thr - 1) / thr)), dim3(thr), 0, nullptr,
+                               (double2*)dcrc.p, (int)d.trg_cel[0], (int)d.trg_cel[1], (int)d.trg_cel[2],
+                               (int)d.src_cel[0], (int)d.src_cel[1], (int)d.src_cel[2], L[0], L[1], L[2], del[0],
+                               del[1], del[2], M[0], M[1], M[2], G, (const double*)dgl.p);
+            SETUP_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)((Lt * 6 + 255) / 256)), dim3(256), 0, nullptr,
+                           (const double2*)dcrc.p, Lt * 6, (int*)dcnt.p);
+        SETUP_HIP(hipGetLastError());
+        SETUP_HIP(run_fwd3<double>(g, tb, dcrc.p, dw1.p, dw2.p, 2, nullptr));
+        hipLaunchKernelGGL(k_gather, dim3((unsigned)((Ut * 48 + 255) / 256)), dim3(256), 0, nullptr,
+                           (double2*)dfur.p, (const double2*)dw2.p, B[0], B[1], B[2], (int)d.fur_ext[0],
+                           (int)d.fur_ext[1], (int)d.fur_ext[2], pair, P);
+        SETUP_HIP(hipGetLastError());
+    }
     int bad = 0;
     SETUP_HIP(hipMemcpy(&bad, dcnt.p, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) {
         err = "Computed circulant contains non-numeric values.";
         return GLA_ERR_NONFINITE;
     }
-    fur.resize((size_t)(Ut * 48));
+    fur.resize((size_t)(Ut * 48 * P));
     SETUP_HIP(hipMemcpy(fur.data(), dfur.p, fur.size() * sizeof(double2), hipMemcpyDeviceToHost));
     return GLA_OK;
 #undef SETUP_HIP

@@ -224,13 +224,16 @@ class GlaOprMem:
 
     @property
     def egoFur(self):
-        """The 8 Fourier branches, each (U1,U2,U3,6) column-major."""
+        """The 8 Fourier branches, each (U1,U2,U3,6) column-major, or
+        (U1,U2,U3,6,Ps,Pt) over the (source, target) partition pairs of an
+        operator whose cell scales differ."""
         U = tuple(self._dims.fur_ext)
-        n = 8 * U[0] * U[1] * U[2] * 6
-        buf = np.empty(n, dtype=self.setTyp)
-        capi.check(capi.lib().gla_opr_export_fur(self._h, buf.ctypes.data_as(ctypes.c_void_p), n))
-        per = U[0] * U[1] * U[2] * 6
-        return [buf[b * per:(b + 1) * per].reshape(U + (6,), order="F") for b in range(8)]
+        Ps, Pt = int(np.prod(self._dims.src_div)), int(np.prod(self._dims.trg_div))
+        per = U[0] * U[1] * U[2] * 6 * Ps * Pt
+        buf = np.empty(8 * per, dtype=self.setTyp)
+        capi.check(capi.lib().gla_opr_export_fur(self._h, buf.ctypes.data_as(ctypes.c_void_p), 8 * per))
+        shp = U + (6,) + ((Ps, Pt) if Ps * Pt > 1 else ())
+        return [buf[b * per:(b + 1) * per].reshape(shp, order="F") for b in range(8)]
 
     def save(self, path):
         """Write the Green data and the geometry it belongs to (the library's
@@ -295,9 +298,13 @@ def _colmajor_torch(x):
 def egoOpr_(egoMem, actVec):
     """egoOpr! (src/glaAct.jl:33-49): apply G to actVec of shape
     (srcCel..., 3) (or (srcCel..., 3, nrhs)); returns a new (trgCel..., 3)
-    array.  Unlike the reference, actVec is left untouched."""
-    S = tuple(egoMem._dims.src_cel)
-    T = tuple(egoMem._dims.trg_cel)
+    array.  Unlike the reference, actVec is left untouched.  For cell scales
+    that differ the shapes are those of the whole volumes (partition cells
+    times divisors): the partitions are gathered and merged inside the
+    library (genPrt / mrgPrt, src/glaActSup.jl:4-18, 939-953)."""
+    d = egoMem._dims
+    S = tuple(int(c) * int(v) for c, v in zip(d.src_cel, d.src_div))
+    T = tuple(int(c) * int(v) for c, v in zip(d.trg_cel, d.trg_div))
     shp = tuple(actVec.shape)
     if shp[:4] != S + (3,) or len(shp) not in (4, 5):
         raise GlaError(capi.GLA_ERR_SIZE, "Size of input vector does not match size of source volume. "

@@ -70,10 +70,13 @@ typedef struct {
 
 typedef struct gla_opr_s* gla_opr_t;
 
-/* Geometry of an operator as the matvec sees it (one source and one target
- * partition): sizes, branch lengths B = (S+T)/2 (src/glaAct.jl:68), the
- * stored Green extents U (src/glaMemSup.jl:121-134) and the per-axis
- * symmetric-storage flags. */
+/* Geometry of an operator as the matvec sees it: the partition sizes
+ * src_cel / trg_cel and divisors src_div / trg_div of GlaExtInf
+ * (src/glaMem.jl:190-216; one partition per side when the cell scales are
+ * equal), branch lengths B = (S+T)/2 of a partition pair (src/glaAct.jl:68),
+ * the stored Green extents U (src/glaMemSup.jl:121-134) and the per-axis
+ * symmetric-storage flags.  A whole volume has cel * div cells per axis; the
+ * Green data holds one block per (source, target) partition pair. */
 typedef struct {
     int64_t src_cel[3], trg_cel[3];
     int64_t brn[3];
@@ -83,6 +86,7 @@ typedef struct {
     int32_t adj_mod;
     int32_t self_opr;
     int64_t dev_brn[3];   /* branch lengths actually transformed on the GPU */
+    int64_t src_div[3], trg_div[3];
 } gla_dims;
 
 /* ---------------------------------------------------------------------- */
@@ -96,17 +100,27 @@ gla_status gla_init(int32_t device);
 
 /* Geometry that gla_opr_create_from_fur expects for (trg, src): the reference
  * derives it in GlaOprMem (src/glaMemSup.jl:16-156).  src == NULL means a
- * self operator.  Fails with GLA_ERR_SCALE for cell scales that are not
- * equal (the multi-partition form, src/glaMem.jl:190-216, is not
- * implemented) and GLA_ERR_SIZE for an odd S+T on an axis. */
+ * self operator.  An external pair whose cell counts sum to an odd number on
+ * some axis is first regenerated with gla_vol_even_gen, as GlaOprMem does
+ * (src/glaMemSup.jl:34-40).  Cell scales that differ by an integer factor
+ * per axis give partitions (GlaExtInf, src/glaMem.jl:190-216): every volume
+ * must then divide into whole partitions, the partition cells of source and
+ * target must sum to an even number per axis (GLA_ERR_SIZE otherwise), and
+ * scales that share no common grid fail with GLA_ERR_SCALE. */
 gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* out);
+
+/* glaVolEveGen (src/glaMem.jl:152-176): a volume with an odd cell count on
+ * some axis gets one more cell there, the cell (and grid) size shrinking so
+ * that the extent is kept; an all-even volume is copied unchanged. */
+gla_status gla_vol_even_gen(const gla_vol* in, gla_vol* out);
 
 /* Operator from precomputed Fourier Green data: the egoFur= re-entry of
  * GlaOprMem (src/glaMemSup.jl:16-20, 54, 158-164).  fur holds the 8 branches
- * back to back, each (U1,U2,U3,6) column-major with components
- * [xx,yy,zz,xy,xz,yz]; fur_dtype gives its element type; fur_len is the
- * number of complex elements (8*U1*U2*U3*6).  Non-finite values fail with
- * GLA_ERR_NONFINITE as the reference's check does. */
+ * back to back, each (U1,U2,U3,6,Ps,Pt) column-major with components
+ * [xx,yy,zz,xy,xz,yz] and Ps / Pt the source / target partition counts
+ * (1 and 1 for equal cell scales); fur_dtype gives its element type; fur_len
+ * is the number of complex elements (8*U1*U2*U3*6*Ps*Pt).  Non-finite values
+ * fail with GLA_ERR_NONFINITE as the reference's check does. */
 gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, const gla_vol* src,
                                    const void* fur, int32_t fur_dtype, int64_t fur_len,
                                    gla_opr_t* out);
@@ -119,7 +133,7 @@ gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vo
 
 /* Copy out the Fourier Green data in the reference's layout (for caching, the
  * preload-file equivalent of examples/GilaOperators.jl:28-55). buf receives
- * 8*U1*U2*U3*6 elements of the operator's dtype; len is its capacity. */
+ * 8*U1*U2*U3*6*Ps*Pt elements of the operator's dtype; len is its capacity. */
 gla_status gla_opr_export_fur(gla_opr_t h, void* buf, int64_t len);
 
 gla_status gla_opr_get_dims(gla_opr_t h, gla_dims* out);
@@ -132,7 +146,11 @@ gla_status gla_opr_adjoint(gla_opr_t h, gla_opr_t* out);
 gla_status gla_opr_destroy(gla_opr_t h);
 
 /* Matvec, egoOpr! (src/glaAct.jl:33-49) on device memory.
- * x: (S1,S2,S3,3,nrhs), y: (T1,T2,T3,3,nrhs), both device pointers of the
+ * x: (S1,S2,S3,3,nrhs), y: (T1,T2,T3,3,nrhs) over the whole source and
+ * target volumes (partition cells times divisors; a partitioned operator
+ * gathers the source partitions, sums the leaves of every partition pair and
+ * scatters the target partitions, genPrt / mrgPrt, src/glaActSup.jl:4-18,
+ * 939-953, src/glaAct.jl:158-164), both device pointers of the
  * operator's dtype, caller-owned; x is not modified.  nx / ny give the
  * number of complex elements of each buffer (a mismatch is GLA_ERR_SIZE, the
  * reference's size check).  The call is asynchronous on `stream` (a
@@ -227,6 +245,24 @@ gla_status gla_opr_load(const char* path, int32_t dtype, int32_t device, gla_opr
  * GLA_ERR_ARG instead of growing them: reserve before capturing.  No
  * counterpart in the reference (its device path allocates per call). */
 gla_status gla_opr_reserve(gla_opr_t h, int64_t nrhs);
+
+/* ---- Recorded batches of operators ------------------------------------
+ * No counterpart in the reference (it applies one operator at a time).  A
+ * fixed batch -- operator ops[i] applied to x[i] -> y[i] with nrhs[i] right-
+ * hand sides (nrhs may be NULL for 1 each), device pointers -- is recorded
+ * once into a HIP graph whose work is dealt over `lanes` parallel branches,
+ * so independent operators too small to fill the GPU run side by side; each
+ * gla_batch_launch replays the whole batch with one graph launch on `stream`
+ * (asynchronous).  The operators' work buffers are grown before recording and
+ * pinned while the batch exists (an apply that would grow them fails); the
+ * operators and vectors must outlive the batch, and an operator listed twice
+ * runs its entries one after the other (they share its work buffers).  All
+ * operators must be on one device. */
+typedef struct gla_batch_s* gla_batch_t;
+gla_status gla_batch_create(const gla_opr_t* ops, const void* const* x, void* const* y, const int64_t* nrhs,
+                            int32_t n, int32_t lanes, gla_batch_t* out);
+gla_status gla_batch_launch(gla_batch_t b, void* stream);
+gla_status gla_batch_destroy(gla_batch_t b);
 
 /* Bytes of device work memory the handle holds for nrhs right-hand sides. */
 int64_t gla_opr_workspace_bytes(gla_opr_t h, int64_t nrhs);

@@ -16,6 +16,11 @@
  *     indSel :647-654, indFlp :658-661).
  *   - gor_crc_ext: circulant over target-minus-source separations as in
  *     egoFunExt! / grdSel (src/glaGen.jl:169-189, 632-643).
+ *   - gor_crc_prt / gor_tensor_mixed: the circulant of one (source, target)
+ *     partition pair of an operator whose cell scales differ: genEgoExt! on
+ *     the partition volumes of GlaOprMem (src/glaMemSup.jl:64-89), all 36
+ *     face pairs of egoFunOut! (src/glaGen.jl:233-246) between cells of
+ *     different sizes with the srfScl factors (src/glaInt.jl:284-305).
  *   - gor_brute_crc: independent direct circular sum y[t] = sum_s c[t-s] x[s]
  *     over the real-space circulant, used to pin the restatement.
  *   - gor_toe_model: test Green tensors (a point-collocation dyadic model,
@@ -38,7 +43,14 @@
  * error (<= 1e-10 self and edge, <= 2e-9 vertex), the tensor at the eight
  * touching offsets (the correction tables, anisotropic cells included) and at
  * regular offsets (<= 1e-11), plus the order-32 vs order-40 convergence of
- * test/intConTest.jl.  The reference's hcubature values of the regular face
+ * test/intConTest.jl.  The mixed-size tensor (tests/test_oracle_partition.py)
+ * is pinned by target and source subdivision: a cell of twice the size on
+ * every axis is the sum of its 8 subcells (the face-pair sum is additive
+ * over a subdivision, interior faces cancel, and srfScl reduces to 1 / V_T
+ * for cells of one shape), so it must equal the corresponding sum of
+ * equal-size model-3 tensors; with equal sizes it reduces to model 3
+ * itself.  The srfScl factors of cells of different shape have no such
+ * check ("parity unpinned" beyond the table itself).  The reference's hcubature values of the regular face
  * pairs (third-party, tolerance 1e-6, src/glaGen.jl:574-576) cannot be run
  * here: model 3 integrates those pairs to their limit instead, so its
  * agreement with the reference is bounded by that tolerance.
@@ -413,15 +425,20 @@ static void srf_sum(const zc *m, zc out[9])
 /* one face pair (target face ft, source face fs, facPar order) over [0,1]^4
  * with an order-q tensor Gauss-Legendre rule, times srfScl: the egoSrfAdp!
  * value (src/glaGen.jl:559-598) with a fixed rule instead of hcubature */
-static zc pair_gl(const double F[6][4][3], int ft, int fs, const double sep[3], const double s[3], zc zeta,
-                  int q, const double *gx, const double *gw)
+/* target faces FT of a cell of sizes sT, source faces FS of sizes sS; the
+ * srfScl factor (src/glaInt.jl:284-305) is indexed (srcFId - 1) * 6 + trgFId
+ * while facPar puts the target face first (src/glaInt.jl:271-280), so the
+ * factor of pair (ft, fs) is the sS-area of face ft over sT along the normal
+ * of fs */
+static zc pair_gl2(const double FT[6][4][3], const double FS[6][4][3], int ft, int fs, const double sep[3],
+                   const double sT[3], const double sS[3], zc zeta, int q, const double *gx, const double *gw)
 {
-    const double scale = ((ft < 2) ? s[1] * s[2] : (ft < 4 ? s[0] * s[2] : s[0] * s[1])) /
-                         ((fs < 2) ? s[0] : (fs < 4 ? s[1] : s[2]));
+    const double scale = ((ft < 2) ? sS[1] * sS[2] : (ft < 4 ? sS[0] * sS[2] : sS[0] * sS[1])) /
+                         ((fs < 2) ? sT[0] : (fs < 4 ? sT[1] : sT[2]));
     double et1[3], et2[3], es1[3], es2[3];
     for (int d = 0; d < 3; ++d) {
-        et1[d] = F[ft][1][d] - F[ft][0][d]; et2[d] = F[ft][3][d] - F[ft][0][d];
-        es1[d] = F[fs][1][d] - F[fs][0][d]; es2[d] = F[fs][3][d] - F[fs][0][d];
+        et1[d] = FT[ft][1][d] - FT[ft][0][d]; et2[d] = FT[ft][3][d] - FT[ft][0][d];
+        es1[d] = FS[fs][1][d] - FS[fs][0][d]; es2[d] = FS[fs][3][d] - FS[fs][0][d];
     }
     zc acc = 0;
     for (int a = 0; a < q; ++a)
@@ -430,8 +447,8 @@ static zc pair_gl(const double F[6][4][3], int ft, int fs, const double sep[3], 
                 for (int e = 0; e < q; ++e) {
                     double r2 = 0;
                     for (int d = 0; d < 3; ++d) {
-                        double dv = (F[ft][0][d] + gx[c] * et1[d] + gx[e] * et2[d]) -
-                                    (F[fs][0][d] + gx[a] * es1[d] + gx[b] * es2[d]) + sep[d];
+                        double dv = (FT[ft][0][d] + gx[c] * et1[d] + gx[e] * et2[d]) -
+                                    (FS[fs][0][d] + gx[a] * es1[d] + gx[b] * es2[d]) + sep[d];
                         r2 += dv * dv;
                     }
                     double r = sqrt(r2);
@@ -439,6 +456,12 @@ static zc pair_gl(const double F[6][4][3], int ft, int fs, const double sep[3], 
                     acc += gw[a] * gw[b] * gw[c] * gw[e] * g;
                 }
     return acc * scale;
+}
+
+static zc pair_gl(const double F[6][4][3], int ft, int fs, const double sep[3], const double s[3], zc zeta,
+                  int q, const double *gx, const double *gw)
+{
+    return pair_gl2(F, F, ft, fs, sep, s, s, zeta, q, gx, gw);
 }
 
 /* Model 3, the reference's construction: egoFunInn! (all 36 face pairs,
@@ -582,6 +605,72 @@ int gor_crc_ext(int32_t model, const int64_t S[3], const int64_t T[3], const dou
             sep[d] = delta[d] + (double)(n[d] < T[d] ? n[d] : n[d] - L[d]) * scl[d];
         tensor_at(model, sep, scl, zeta, qf, qn, dst);
     }
+    return 0;
+}
+
+/* Separated cells of different sizes (a partitioned operator's pair,
+ * genEgoExt! with partition volumes, src/glaMemSup.jl:64-89): all 36 face
+ * pairs of egoFunOut! (src/glaGen.jl:233-246) between the faces of a target
+ * cell of sizes sT and a source cell of sizes sS, srfScl(sT, sS) scaling,
+ * srfSum!.  The order follows the separation in mean cell sizes. */
+static void tensor_mixed(const double sep[3], const double sT[3], const double sS[3], zc zeta, zc out[9])
+{
+    double FT[6][4][3], FS[6][4][3], mid[3], hm = 0;
+    cub_fac(sT, FT);
+    cub_fac(sS, FS);
+    for (int d = 0; d < 3; ++d) {
+        mid[d] = 0.5 * (sT[d] + sS[d]);
+        hm = fmax(hm, fmax(sT[d], sS[d]));
+    }
+    const int q = ref_order(sep, mid, zeta);
+    const int qp = 6 + (int)ceil(6.0 * 2.0 * M_PI * cabs(zeta) * hm);
+    const int qq = q > qp ? q : (qp > 20 ? 20 : qp);
+    double gx[24], gw[24];
+    gl_rule(qq, gx, gw);
+    zc srf[36];
+    for (int fp = 0; fp < 36; ++fp)
+        srf[fp] = pair_gl2(FT, FS, fp / 6, fp % 6, sep, sT, sS, zeta, qq, gx, gw);
+    srf_sum(srf, out);
+}
+
+/* Circulant of one (source, target) partition pair: as gor_crc_ext with
+ * target cells sT, source cells sS and the separation lattice step M (the
+ * coarser cell size per axis, the grid step of both partition volumes,
+ * sepGrd, src/glaGen.jl:603-629). */
+int gor_crc_prt(const int64_t S[3], const int64_t T[3], const double sT[3], const double sS[3], const double M[3],
+                const double delta[3], double zr, double zi, zc *crc)
+{
+    const zc zeta = zr + I * zi;
+    int64_t L[3];
+    for (int d = 0; d < 3; ++d) { if ((S[d] + T[d]) & 1) return 1; L[d] = S[d] + T[d]; }
+    const int64_t Lt = L[0] * L[1] * L[2];
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t p = 0; p < Lt; ++p) {
+        int64_t n[3] = {p % L[0], (p / L[0]) % L[1], p / (L[0] * L[1])};
+        zc *dst = crc + 9 * p;
+        if (n[0] == T[0] || n[1] == T[1] || n[2] == T[2]) {
+            for (int e = 0; e < 9; ++e) dst[e] = 0;
+            continue;
+        }
+        double sep[3];
+        for (int d = 0; d < 3; ++d)
+            sep[d] = delta[d] + (double)(n[d] < T[d] ? n[d] : n[d] - L[d]) * M[d];
+        tensor_mixed(sep, sT, sS, zeta, dst);
+    }
+    return 0;
+}
+
+/* Model 3 at one separation (3x3, column-major) */
+int gor_tensor_ref(const double sep[3], const double s[3], double zr, double zi, zc *out)
+{
+    tensor_ref(sep, s, zr + I * zi, out);
+    return 0;
+}
+
+/* The tensor between one target and one source cell (3x3, column-major) */
+int gor_tensor_mixed(const double sep[3], const double sT[3], const double sS[3], double zr, double zi, zc *out)
+{
+    tensor_mixed(sep, sT, sS, zr + I * zi, out);
     return 0;
 }
 

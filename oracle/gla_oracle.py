@@ -38,6 +38,9 @@ def lib():
             "gor_crc_self": [vp, vp, vp],
             "gor_toe_model": [i32, vp, vp, d, d, i32, i32, vp],
             "gor_crc_ext": [i32, vp, vp, vp, vp, d, d, i32, i32, vp],
+            "gor_crc_prt": [vp, vp, vp, vp, vp, vp, d, d, vp],
+            "gor_tensor_mixed": [vp, vp, vp, d, d, vp],
+            "gor_tensor_ref": [vp, vp, d, d, vp],
             "gor_fft_axis_c128": [vp, vp, i64, i32, i32],
             "gor_gl_rule": [i32, vp, vp],
             "gor_gl_pm1": [i32, vp, vp],
@@ -130,6 +133,97 @@ def crc_ext(S, T, scl, delta, model=0, zeta=1.0 + 0j, qf=4, qn=8, int_ord=32):
                          _p(np.asarray(delta, np.float64)), zeta.real, zeta.imag, int(qf), int(qn), _p(crc)):
         raise ValueError("S+T must be even on every axis")
     return crc
+
+
+def tensor_mixed(sep, sT, sS, zeta=1.0 + 0j):
+    """3x3 tensor between a target cell of sizes sT and a separated source
+    cell of sizes sS at separation sep (egoFunOut!, src/glaGen.jl:233-246)."""
+    out = np.zeros((3, 3), dtype=np.complex128, order="F")
+    f64 = lambda v: np.ascontiguousarray(np.asarray(v, np.float64))
+    lib().gor_tensor_mixed(_p(f64(sep)), _p(f64(sT)), _p(f64(sS)), zeta.real, zeta.imag, _p(out))
+    return out
+
+
+def tensor_ref(sep, scl, zeta=1.0 + 0j, int_ord=32):
+    """Model-3 tensor (3x3) at one separation of equal cells."""
+    _use_singular(3, scl, zeta, int_ord)
+    out = np.zeros((3, 3), dtype=np.complex128, order="F")
+    f64 = lambda v: np.ascontiguousarray(np.asarray(v, np.float64))
+    lib().gor_tensor_ref(_p(f64(sep)), _p(f64(scl)), zeta.real, zeta.imag, _p(out))
+    return out
+
+
+def crc_prt(S, T, sT, sS, M, delta, zeta=1.0 + 0j):
+    """Circulant (3,3,S+T) of one partition pair: separations delta + n M on
+    the lattice of step M, target cells sT, source cells sS."""
+    crc = np.zeros((3, 3) + tuple(s + t for s, t in zip(S, T)), dtype=np.complex128, order="F")
+    f64 = lambda v: np.ascontiguousarray(np.asarray(v, np.float64))
+    if lib().gor_crc_prt(_p(_i64(S)), _p(_i64(T)), _p(f64(sT)), _p(f64(sS)), _p(f64(M)), _p(f64(delta)),
+                         zeta.real, zeta.imag, _p(crc)):
+        raise ValueError("S+T must be even on every axis")
+    return crc
+
+
+def partitions(div):
+    """Offsets of the partitions of a divisor grid, x fastest (the
+    CartesianIndices order of GlaExtInf's srcPar / trgPar,
+    src/glaMem.jl:211-212)."""
+    return [(i, j, k) for k in range(div[2]) for j in range(div[1]) for i in range(div[0])]
+
+
+def partition_fur(Tfull, Sfull, sT, sS, orgT, orgS, zeta=1.0 + 0j):
+    """Fourier Green data of a partitioned external operator in the
+    reference layout (8 branches of (B1,B2,B3,6,Ps,Pt), concatenated flat):
+    per (source, target) partition pair the circulant of genEgoExt! on the
+    partition volumes (src/glaMemSup.jl:64-89), then the Fourier storage of
+    src/glaMemSup.jl:96-156 without symmetric truncation.  Returns (fur,
+    S, T, srcDiv, trgDiv) with S, T the partition cell counts."""
+    M = [max(a, b) for a, b in zip(sT, sS)]
+    tdiv = [int(round(m / a)) for m, a in zip(M, sT)]
+    sdiv = [int(round(m / b)) for m, b in zip(M, sS)]
+    T = [c // v for c, v in zip(Tfull, tdiv)]
+    S = [c // v for c, v in zip(Sfull, sdiv)]
+    t0 = [o - h * (c - 1) / 2 for o, h, c in zip(orgT, sT, Tfull)]
+    s0 = [o - h * (c - 1) / 2 for o, h, c in zip(orgS, sS, Sfull)]
+    blocks = []
+    for ot in partitions(tdiv):
+        for os_ in partitions(sdiv):
+            delta = [a + i * h - b - j * g for a, i, h, b, j, g in zip(t0, ot, sT, s0, os_, sS)]
+            crc = crc_prt(S, T, sT, sS, M, delta, zeta)
+            blocks.append(fur_from_crc(S, T, (0, 0, 0), crc))
+    B, U = dims(S, T, (0, 0, 0))
+    per = U[0] * U[1] * U[2] * 6
+    P = len(blocks)
+    fur = np.zeros(8 * per * P, dtype=np.complex128)
+    for p, f in enumerate(blocks):
+        for b in range(8):
+            fur[(b * P + p) * per:(b * P + p + 1) * per] = f[b * per:(b + 1) * per]
+    return fur, tuple(S), tuple(T), tuple(sdiv), tuple(tdiv)
+
+
+def partition_matvec(S, T, sdiv, tdiv, fur, x, dtype=np.complex128):
+    """The partitioned matvec: source partitions gathered from the whole
+    source field x (genPrtHst, src/glaActSup.jl:4-18), every (source,
+    target) pair's leaves summed into its target partition (egoBrnHst!,
+    src/glaAct.jl:158-164 -- the sum commutes with the inverse transforms)
+    and the target partitions merged into the whole target field (mrgPrtHst,
+    src/glaActSup.jl:939-953).  x is (S*sdiv..., 3) column-major."""
+    x = np.asarray(x)
+    Tf = tuple(t * v for t, v in zip(T, tdiv))
+    y = np.zeros(Tf + (3,), dtype=dtype, order="F")
+    B, U = dims(S, T, (0, 0, 0))
+    per = U[0] * U[1] * U[2] * 6
+    sp, tp = partitions(sdiv), partitions(tdiv)
+    P = len(sp) * len(tp)
+    for it, ot in enumerate(tp):
+        acc = np.zeros(tuple(T) + (3,), dtype=dtype, order="F")
+        for js, os_ in enumerate(sp):
+            p = js + len(sp) * it
+            f = np.concatenate([fur[(b * P + p) * per:(b * P + p + 1) * per] for b in range(8)])
+            xs = x[os_[0]::sdiv[0], os_[1]::sdiv[1], os_[2]::sdiv[2], :]
+            acc += matvec(S, T, (0, 0, 0), f, xs, dtype=dtype)
+        y[ot[0]::tdiv[0], ot[1]::tdiv[1], ot[2]::tdiv[2], :] = acc
+    return y
 
 
 def fur_from_crc(S, T, sym, crc):

@@ -46,11 +46,60 @@ def test_dims_errors(gila):
     L = gila.lib()
     d = _capi.gla_dims()
     a = gila.GlaVol((4, 4, 4), ((1, 32),) * 3)._c()
+    # an odd cell sum regenerates the odd volume (3 cells of 1/32 -> 4 of
+    # 3/128, src/glaMemSup.jl:34-40), whose scale then shares no grid with
+    # 1/32 (src/glaMem.jl:192-195)
     b = gila.GlaVol((3, 4, 4), ((1, 32),) * 3, (1, 0, 0))._c()
-    assert L.gla_opr_dims_for(ctypes.byref(a), ctypes.byref(b), ctypes.byref(d)) == _capi.GLA_ERR_SIZE
+    assert L.gla_opr_dims_for(ctypes.byref(a), ctypes.byref(b), ctypes.byref(d)) == _capi.GLA_ERR_SCALE
     c = gila.GlaVol((4, 4, 4), ((1, 33),) * 3, (1, 0, 0))._c()
     assert L.gla_opr_dims_for(ctypes.byref(a), ctypes.byref(c), ctypes.byref(d)) == _capi.GLA_ERR_SCALE
     assert "common scale" in L.gla_last_error().decode()
+
+
+def test_odd_sum_regenerates_volumes(gila):
+    from gila_amd import _capi
+    L = gila.lib()
+    v = gila.GlaVol((3, 4, 5), ((1, 32),) * 3)._c()
+    w = _capi.gla_vol()
+    _capi.check(L.gla_vol_even_gen(ctypes.byref(v), ctypes.byref(w)))
+    assert tuple(w.cel) == (4, 4, 6)
+    assert [(w.scl_num[a], w.scl_den[a]) for a in range(3)] == [(3, 128), (1, 32), (5, 192)]
+    # 3 cells of 1/32 become 4 of 3/128, the scale of the other volume
+    t = gila.GlaVol((3, 2, 2), ((1, 32), (3, 128), (3, 128)))._c()
+    s = gila.GlaVol((4, 2, 2), ((3, 128),) * 3, (1, 0, 0))._c()
+    d = _capi.gla_dims()
+    _capi.check(L.gla_opr_dims_for(ctypes.byref(t), ctypes.byref(s), ctypes.byref(d)))
+    assert tuple(d.trg_cel) == (4, 2, 2) and tuple(d.brn) == (4, 2, 2)
+
+
+def test_partition_dims(gila):
+    """GlaExtInf (src/glaMem.jl:190-216): the finer volume splits into
+    ratio-strided partitions of the coarser lattice."""
+    from gila_amd import _capi
+    L = gila.lib()
+    d = _capi.gla_dims()
+    t = gila.GlaVol((4, 6, 4), ((1, 16), (1, 32), (1, 32)))._c()
+    s = gila.GlaVol((8, 6, 4), ((1, 32),) * 3, (1, 0, 0))._c()
+    _capi.check(L.gla_opr_dims_for(ctypes.byref(t), ctypes.byref(s), ctypes.byref(d)))
+    assert tuple(d.src_div) == (2, 1, 1) and tuple(d.trg_div) == (1, 1, 1)
+    assert tuple(d.src_cel) == (4, 6, 4) and tuple(d.trg_cel) == (4, 6, 4)
+    assert tuple(d.brn) == (4, 6, 4) and tuple(d.sym) == (0, 0, 0) and tuple(d.fur_ext) == (4, 6, 4)
+    _capi.check(L.gla_opr_dims_for(ctypes.byref(s), ctypes.byref(t), ctypes.byref(d)))
+    assert tuple(d.trg_div) == (2, 1, 1) and tuple(d.src_div) == (1, 1, 1)
+    # source cells not a multiple of the ratio
+    s8 = gila.GlaVol((8, 6, 4), ((1, 32),) * 3, (1, 0, 0))._c()
+    t3 = gila.GlaVol((4, 6, 4), ((3, 32), (1, 32), (1, 32)))._c()
+    assert L.gla_opr_dims_for(ctypes.byref(t3), ctypes.byref(s8), ctypes.byref(d)) == _capi.GLA_ERR_SIZE
+    assert "multiple of the scale ratio" in L.gla_last_error().decode()
+    # partition cells with an odd sum
+    s6 = gila.GlaVol((6, 6, 4), ((1, 32),) * 3, (1, 0, 0))._c()
+    assert L.gla_opr_dims_for(ctypes.byref(t), ctypes.byref(s6), ctypes.byref(d)) == _capi.GLA_ERR_SIZE
+    assert "partition" in L.gla_last_error().decode()
+    # a volume smaller than one coarse cell
+    s1 = gila.GlaVol((1, 6, 4), ((1, 32),) * 3, (1, 0, 0))._c()
+    t1 = gila.GlaVol((2, 6, 4), ((1, 16), (1, 32), (1, 32)))._c()
+    assert L.gla_opr_dims_for(ctypes.byref(t1), ctypes.byref(s1), ctypes.byref(d)) == _capi.GLA_ERR_SCALE
+    assert "smaller than cell" in L.gla_last_error().decode()
 
 
 def test_non_complex_settyp_raises_argument_error(gila):

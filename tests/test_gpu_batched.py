@@ -149,3 +149,67 @@ def test_apply_under_capture_needs_reserved_buffers(gila, gpu):
     g2.replay()
     torch.cuda.synchronize()
     assert torch.equal(yd, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_recorded_batch_equals_eager(gila, gpu, lanes):
+    """gla_batch_create / gla_batch_launch: the mixed batch of 16 operators
+    recorded once over `lanes` parallel branches; every replay (with eager
+    work and synchronizations between replays) equals the eager sequence bit
+    for bit; an operator listed twice shares its work buffers safely."""
+    import torch
+    from gila_amd import _capi, inputs
+    shapes = [(24, 24, 24), (32, 48, 64), (64, 64, 64), (80, 40, 40)]
+    uu = inputs.uniform(4242, 16)
+    mems, xs, ys = [], [], []
+    for i in range(16):
+        cells = shapes[int(uu[i] * len(shapes))]
+        mems.append(gila.GlaOprMem(gila.GlaKerOpt(True), gila.GlaVol(cells, H), setTyp=np.complex64))
+        xs.append(torch.from_numpy(inputs.complex_gaussian(cells + (3,), seed=500 + i).astype(np.complex64)
+                                   .reshape(-1, order="F").copy()).cuda())
+        ys.append(torch.empty_like(xs[-1]))
+    # the first operator again, on another input
+    mems.append(mems[0])
+    xs.append(torch.flip(xs[0], [0]).contiguous())
+    ys.append(torch.empty_like(xs[0]))
+    print(f"case recorded batch of {len(mems)} operators, {lanes} lanes", flush=True)
+    st = torch.cuda.current_stream()
+    L = gila.lib()
+
+    def eager():
+        for m, x, y in zip(mems, xs, ys):
+            _apply(gila, m, x, y, 1, st)
+    eager()
+    torch.cuda.synchronize()
+    ref = [y.clone() for y in ys]
+    n = len(mems)
+    ops = (ctypes.c_void_p * n)(*[m._h for m in mems])
+    xp = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+    yp = (ctypes.c_void_p * n)(*[y.data_ptr() for y in ys])
+    b = ctypes.c_void_p()
+    _capi.check(L.gla_batch_create(ops, xp, yp, None, n, lanes, ctypes.byref(b)))
+    try:
+        for rep in range(3):
+            for y in ys:
+                y.zero_()
+            eager()
+            torch.cuda.synchronize()
+            for y in ys:
+                y.zero_()
+            torch.cuda.synchronize()
+            _capi.check(L.gla_batch_launch(b, ctypes.c_void_p(st.cuda_stream)))
+            torch.cuda.synchronize()
+            for k, (y, r) in enumerate(zip(ys, ref)):
+                assert torch.equal(y, r), (rep, k)
+        # the pinned work buffers cannot grow while the batch exists
+        m = mems[1]
+        big = torch.zeros(2 * xs[1].numel(), dtype=torch.complex64, device="cuda")
+        out = torch.empty_like(big)
+        st_ = L.gla_opr_apply(m._h, ctypes.c_void_p(big.data_ptr()), big.numel(), ctypes.c_void_p(out.data_ptr()),
+                              out.numel(), 2, ctypes.c_void_p(st.cuda_stream))
+        assert st_ == _capi.GLA_ERR_ARG and "batch" in L.gla_last_error().decode()
+    finally:
+        _capi.check(L.gla_batch_destroy(b))
+    _apply(gila, mems[1], big, out, 2, st)
+    torch.cuda.synchronize()
