@@ -377,7 +377,10 @@ def serve_leg(gila, a, log):
         # refinement against plain FP64)
         agree = max(float(np.linalg.norm(r2.result - r1.result) / np.linalg.norm(r1.result))
                     for r1, r2 in zip(reqs, reqs2))
+        span_arr = float(arrivals[-1] - arrivals[0]) if nreq > 1 else 0.0
         sweep.append({"rate_x_sequential": mult, "requests": nreq, "arrival_rate_per_s": lam,
+                      "arrival_span_s": span_arr,
+                      "realised_arrival_rate_per_s": (nreq - 1) / span_arr if span_arr > 0 else None,
                       "solves_per_s": st["solves_per_s"],
                       "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
                       "speedup_vs_sequential_fp64": st["solves_per_s"] / rate_seq,
