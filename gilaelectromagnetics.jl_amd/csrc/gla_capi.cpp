@@ -364,7 +364,7 @@ gla_status upload(gla_opr_t h, const std::vector<std::complex<double>>& furd)
     }
     const int Bv[3] = {(int)d.dev_brn[0], (int)d.dev_brn[1], (int)d.dev_brn[2]};
     const int Lv[3] = {2 * Bv[0], 2 * Bv[1], 2 * Bv[2]};
-    const std::vector<int> order = gla::zpass_order(Bv, Lv, h->tab.am, gla::zpass_tile_width((int)sizeof(C), Bv[2]));
+    const std::vector<int> order = gla::zpass_order(Bv, Lv, h->tab.am, gla::zpass_tile_width((int)sizeof(C), Bv[2], Lv[0], Lv[1]));
     auto dev = std::make_shared<DevData>();
     dev->device = h->opts.device;
     GLA_HIP(hipMalloc(&dev->gf, g.size() * sizeof(C)));
@@ -1346,7 +1346,10 @@ bool len_supported(int n)
     return false;
 }
 
-int zpass_tile_width(int complex_bytes, int n) { return ztune(complex_bytes, n).px; }
+int zpass_tile_width(int complex_bytes, int n, int L1, int L2)
+{
+    return ztune(complex_bytes, n, (int64_t)L1 * L2 < ZSMALL_COLS).px;
+}
 
 std::vector<int> zpass_order(const int B[3], const int L[3], const AxisMap am[3], int px)
 {

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gla_tune.hpp"
+
 namespace gla {
 
 template <typename R>
@@ -68,8 +70,11 @@ __host__ __device__ constexpr int smallest_factor(int m)
 
 // Factor N into radices from {16,8,4,2,3,5,other primes}; the first radix is
 // the largest supported radix dividing both N and E, the last the largest
-// dividing what remains and E, the middle ones anything.
-__host__ __device__ constexpr Plan make_plan(int N, int E)
+// dividing what remains and E, the middle ones the largest remaining factors
+// (pol 0: fewest LDS exchanges) or, where one exists, the largest that also
+// divides E (pol 1: a middle stage of radix r runs E / r butterflies per
+// thread, so every thread stays busy, at the cost of more exchanges).
+__host__ __device__ constexpr Plan make_plan(int N, int E, int pol = 0)
 {
     Plan p{N, 0, {}};
     int m = N;
@@ -85,6 +90,9 @@ __host__ __device__ constexpr Plan make_plan(int N, int E)
     if (last) m /= last;
     while (m > 1) {
         int r = 0;
+        if (pol == 1)
+            for (int i = 0; i < 6 && !r; ++i)
+                if (m % pref[i] == 0 && E % pref[i] == 0) r = pref[i];
         for (int i = 0; i < 6 && !r; ++i)
             if (m % pref[i] == 0) r = pref[i];
         if (!r) r = smallest_factor(m);
@@ -305,7 +313,7 @@ struct PencilFft {
     }
     static constexpr int E = N / TP;
     static_assert(N % TP == 0, "TP must divide N");
-    static constexpr Plan P = make_plan(N, E);
+    static constexpr Plan P = make_plan(N, E, ptune((int)sizeof(cpx<R>), N));
     static constexpr bool FIRST_REG = (P.nst > 0) && (E % P.rad[0] == 0);
     static constexpr bool LAST_REG = (P.nst > 1) && (E % P.rad[P.nst - 1] == 0);
 

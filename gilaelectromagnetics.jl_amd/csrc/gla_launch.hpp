@@ -44,9 +44,10 @@ struct DevTab {
     int zper_xcd;          // pass-3 workgroups per XCD (grid = 8 * zper_xcd)
 };
 
-// columns per pass-3 workgroup for the element type (bytes per complex) and
-// the pass-3 transform length (the table of gla_tune.hpp)
-int zpass_tile_width(int complex_bytes, int n);
+// columns per pass-3 workgroup for the element type (bytes per complex), the
+// pass-3 transform length and the grid's L1 x L2 columns (the tables of
+// gla_tune.hpp)
+int zpass_tile_width(int complex_bytes, int n, int L1, int L2);
 
 // Pass-3 tile order: ntile = ceil(L1 / PX) column tiles times L2 rows,
 // padded to a multiple of 8 with -1.  With symmetric storage on axes 1 and 2

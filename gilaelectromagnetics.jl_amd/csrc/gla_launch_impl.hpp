@@ -3,18 +3,8 @@
 #pragma once
 #include "gla_launch.hpp"
 
-// kernel-shape defaults (overridable at compile time for variant timing)
-// Rows of passes 1, 2, 4, 5: target elements per thread by length (same-GPU
-// timing against 8 everywhere, after the row-pass addressing change: 4 is
-// faster up to 128 in both types -- 128^3 ComplexF32 322.5 -> 317.5 us,
-// ComplexF64 651.8 -> 638.7 us per matvec -- and at 160 in ComplexF32, slower
-// at 256 and at 160 in ComplexF64).  GLA_EROW_F / GLA_EROW_D force one value.
-#ifndef GLA_EROW_SMALL_F
-#define GLA_EROW_SMALL_F 160
-#endif
-#ifndef GLA_EROW_SMALL_D
-#define GLA_EROW_SMALL_D 128
-#endif
+// kernel-shape defaults (overridable at compile time for variant timing); the
+// per-length shapes come from the tables of gla_tune.hpp
 // pass-3 schedule: 0 keeps the input column in registers through the even
 // parity, 2 splits both parities first (fewer registers for FP64)
 #ifndef GLA_ZRELOAD_F
@@ -25,13 +15,6 @@
 #endif
 #ifndef GLA_ZBATCH_KB
 #define GLA_ZBATCH_KB 64
-#endif
-// axis-2 passes: consecutive axis-1 columns per workgroup (128-byte rows)
-#ifndef GLA_PXY_F
-#define GLA_PXY_F 16
-#endif
-#ifndef GLA_PXY_D
-#define GLA_PXY_D 8
 #endif
 #ifndef GLA_YLDS_KB
 #define GLA_YLDS_KB 160
@@ -69,9 +52,9 @@ namespace gla {
 // elements per thread: the divisor of N closest to `target` from below,
 // raised until the workgroup fits in `maxtp` threads per pencil
 // LDS exchanges of the radix plan for N with E elements per thread
-__host__ __device__ constexpr int plan_exchanges(int N, int E)
+__host__ __device__ constexpr int plan_exchanges(int N, int E, int pol)
 {
-    const Plan p = make_plan(N, E);
+    const Plan p = make_plan(N, E, pol);
     if (p.nst <= 1) return (p.nst == 1 && E % p.rad[0] == 0) ? 0 : 1;
     const bool fr = E % p.rad[0] == 0, lr = E % p.rad[p.nst - 1] == 0;
     return (p.nst - 1) + (fr ? 0 : 1) + (lr ? 0 : 1);
@@ -82,7 +65,7 @@ __host__ __device__ constexpr int plan_exchanges(int N, int E)
 // inside one wave) -- unless that would leave more than 64 elements per
 // thread (lengths with a large odd factor, e.g. 81, 145), in which case the
 // pencil spans waves (workgroup barriers) with at most 32 elements per thread
-__host__ __device__ constexpr int pick_e_in(int N, int target, int maxtp, bool wave, int emax)
+__host__ __device__ constexpr int pick_e_in(int N, int target, int maxtp, bool wave, int emax, int pol)
 {
     int best = 0, bestscore = 1 << 30;
     for (int e = 1; e <= N; ++e) {
@@ -90,55 +73,36 @@ __host__ __device__ constexpr int pick_e_in(int N, int target, int maxtp, bool w
         const int tp = N / e;
         if (tp > maxtp || e > emax) continue;
         const int d = e > target ? 2 * (e - target) : target - e;
-        const int score = d + 4 * plan_exchanges(N, e) + ((wave && 64 % tp != 0) ? 1000 : 0);
+        const int score = d + 4 * plan_exchanges(N, e, pol) + ((wave && 64 % tp != 0) ? 1000 : 0);
         if (score < bestscore) { bestscore = score; best = e; }
     }
     return best;
 }
 
-__host__ __device__ constexpr int pick_e(int N, int target, int maxtp, bool wave = false)
+__host__ __device__ constexpr int pick_e(int N, int target, int maxtp, bool wave, int pol)
 {
-    const int e = pick_e_in(N, target, maxtp, wave, N);
+    const int e = pick_e_in(N, target, maxtp, wave, N, pol);
     if (e > 0 && e <= 64) return e;
-    const int f = pick_e_in(N, target, maxtp, false, 32);
+    const int f = pick_e_in(N, target, maxtp, false, 32, pol);
     return f > 0 ? f : (e > 0 ? e : N);
 }
 
 template <typename R>
 struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
-    static constexpr int PX = DBL ? GLA_PXY_D : GLA_PXY_F;   // axis-2 passes: columns per bundle
-    // passes 1, 2, 4, 5
-    static constexpr int e_row(int n)
-    {
-#if defined(GLA_EROW_F) && defined(GLA_EROW_D)
-        return DBL ? GLA_EROW_D : GLA_EROW_F;
-#else
-        return n <= (DBL ? GLA_EROW_SMALL_D : GLA_EROW_SMALL_F) ? 4 : 8;
-#endif
-    }
     static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
     static constexpr int ZMINW = DBL ? GLA_ZMINW_D : GLA_ZMINW_F;
     static constexpr int ZGG = DBL ? GLA_ZGG_D : GLA_ZGG_F;
 };
 
-
-template <typename R, int N>
-struct LenK {
-    using C = Cfg<R>;
-    static constexpr int TP_ROW = N / pick_e(N, C::e_row(N), 64, true);
-    static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
-    // axis-2 passes: the bundle width, capped so the two parities' LDS
-    // buffers and the staged tables stay within GLA_YLDS_KB
-    static constexpr int PXY = [] {
-        int px = C::PX;
-        while (px > 1 && (YK<R>::value * px + (GLA_XYTAB ? 2 : 0)) * N * (int)sizeof(cpx<R>) > GLA_YLDS_KB * 1024) px /= 2;
-        return px;
-    }();
-    static constexpr int TP_Y = N / pick_e(N, C::e_row(N), 1024 / PXY);
-    // pass 3: shape from the tuning table (gla_tune.hpp)
-    static constexpr int PXZ = ztune(sizeof(cpx<R>), N).px;
-    static constexpr int TP_Z = N / pick_e(N, ztune(sizeof(cpx<R>), N).e, 1024 / PXZ, GLA_ZWAVE != 0);
+// pass-3 shape of length N for the large-grid (SMALL = 0) or small-grid
+// class of gla_tune.hpp
+template <typename R, int N, int SMALL>
+struct ZK {
+    static constexpr int CB = (int)sizeof(cpx<R>);
+    static constexpr int POL = ptune(CB, N);
+    static constexpr int PXZ = ztune(CB, N, SMALL != 0).px;
+    static constexpr int TP_Z = N / pick_e(N, ztune(CB, N, SMALL != 0).e, 1024 / PXZ, GLA_ZWAVE != 0, POL);
     // pass 3 keeps each pencil inside one wave when its threads divide 64
     static constexpr int WL_Z = (GLA_ZWAVE != 0 && 64 % TP_Z == 0 && (TP_Z * PXZ) % 64 == 0) ? 1 : 0;
     static constexpr int PS_Z = WL_Z ? N + (N >> GLA_ZPADSH) + 1 : N;
@@ -146,13 +110,30 @@ struct LenK {
     // buffers fit the budget and the plan has no direct odd radix above 16
     // (a radix-29 butterfly for three components at once would spill)
     static constexpr int RMAX_Z = [] {
-        const Plan p = make_plan(N, N / TP_Z);
+        const Plan p = make_plan(N, N / TP_Z, POL);
         int m = 1;
         for (int i = 0; i < p.nst; ++i) m = p.rad[i] > m ? p.rad[i] : m;
         return m;
     }();
     static constexpr int KB_Z =
         (RMAX_Z <= 16 && 3 * PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024) ? 3 : 1;
+};
+
+template <typename R, int N>
+struct LenK {
+    static constexpr int CB = (int)sizeof(cpx<R>);
+    static constexpr int POL = ptune(CB, N);
+    static constexpr RShape RS = rtune(CB, N);
+    static constexpr int TP_ROW = N / pick_e(N, RS.ex, 64, true, POL);
+    static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
+    // axis-2 passes: the bundle width, capped so the two parities' LDS
+    // buffers and the staged tables stay within GLA_YLDS_KB
+    static constexpr int PXY = [] {
+        int px = RS.pxy;
+        while (px > 1 && (YK<R>::value * px + (GLA_XYTAB ? 2 : 0)) * N * CB > GLA_YLDS_KB * 1024) px /= 2;
+        return px;
+    }();
+    static constexpr int TP_Y = N / pick_e(N, RS.ey, 1024 / PXY, false, POL);
 };
 
 template <typename R, int N>
@@ -205,10 +186,10 @@ static hipError_t launch_inv_y(const Geom& g, const DevTab& tb, const void* W2, 
     return hipGetLastError();
 }
 
-template <typename R, int N>
-static hipError_t launch_zpass_chunk(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
+template <typename R, int N, int SMALL>
+static hipError_t launch_zpass_shape(const Geom& g, const DevTab& tb, void* W2, int nrhs, hipStream_t st)
 {
-    using K = LenK<R, N>;
+    using K = ZK<R, N, SMALL>;
     constexpr int PX = K::PXZ;
     constexpr int SC = Cfg<R>::ZSCHED;
     dim3 grid(8 * tb.zper_xcd * nrhs, 1, 1);   // right-hand side inside the XCD stretch
@@ -237,7 +218,10 @@ static hipError_t launch_zpass(const Geom& g, const DevTab& tb, void* W2, int nr
     const int maxr = (int)(lim < 1 ? 1 : (lim < nrhs ? lim : nrhs));
     for (int r0 = 0; r0 < nrhs; r0 += maxr) {
         const int cnt = nrhs - r0 < maxr ? nrhs - r0 : maxr;
-        const hipError_t e = launch_zpass_chunk<R, N>(g, tb, (cpx<R>*)W2 + (int64_t)r0 * g.w2s, cnt, st);
+        const bool small = (int64_t)g.L[0] * g.L[1] < ZSMALL_COLS;
+        cpx<R>* const w = (cpx<R>*)W2 + (int64_t)r0 * g.w2s;
+        const hipError_t e = small ? launch_zpass_shape<R, N, 1>(g, tb, w, cnt, st)
+                                   : launch_zpass_shape<R, N, 0>(g, tb, w, cnt, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

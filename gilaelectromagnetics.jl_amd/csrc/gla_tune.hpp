@@ -1,28 +1,40 @@
-// Pass-3 shape per transform length and element type: the target elements per
-// thread (E, the radix plan follows from it) and the columns per workgroup
-// (PX, also the tile width of the host-built tile order).  The table is the
-// result of gilaelectromagnetics.jl_amd/tune_zpass.py (timing of every
-// candidate shape at every length on the target GPU); lengths not listed use
-// the defaults.  Tuning builds force one shape for all lengths with
-// -DGLA_ZT_E_F=.. -DGLA_ZT_PX_F=.. (ComplexF32) / -DGLA_ZT_E_D=.. -DGLA_ZT_PX_D=..
-// The ComplexF32 entry for 128 was re-measured after the pass-3 addressing
-// change (same-GPU timing of the 128^3 operator: {4, 8} 161.6 us against
-// 178.2 us for the default {8, 16}).
+// Launch shapes per transform length, element type and grid class: the
+// tables are the output of gilaelectromagnetics.jl_amd/tune.py (timing of every
+// candidate on the target GPU, written between the BEGIN / END markers);
+// lengths not listed use the defaults.
+//   ztune: pass 3 -- target elements per thread (E, the radix plan follows)
+//          and columns per workgroup (PX, also the tile width of the
+//          host-built tile order), for grids with many columns and for small
+//          cross-sections (L1 L2 < ZSMALL_COLS columns: fewer, narrower
+//          workgroups fill the GPU better there)
+//   rtune: passes 1, 2, 4, 5 -- elements per thread of the axis-1 rows (ex)
+//          and of the axis-2 columns (ey), and the axis-2 bundle width (pxy)
+//   ptune: the radix plan of every pencil transform of that length -- 0 takes
+//          the largest radices (fewest LDS exchanges), 1 only radices that
+//          divide the elements per thread (every thread busy in every stage,
+//          more exchanges)
+// Tuning builds force one value for all lengths:
+//   -DGLA_ZT_E_F=.. -DGLA_ZT_PX_F=.. / -DGLA_ZT_E_D=.. -DGLA_ZT_PX_D=..
+//   -DGLA_RT_EX_F=.. -DGLA_RT_EY_F=.. -DGLA_RT_PXY_F=.. (and _D)
+//   -DGLA_PT_F=.. / -DGLA_PT_D=..
+// (_F: ComplexF32, _D: ComplexF64.)
 #pragma once
 
 namespace gla {
+
+constexpr int ZSMALL_COLS = 128 * 128;
 
 struct ZShape {
     int e, px;
 };
 
-constexpr ZShape ztune(int complex_bytes, int n)
+struct RShape {
+    int ex, ey, pxy;
+};
+
+constexpr ZShape ztune_large(int complex_bytes, int n)
 {
     if (complex_bytes == 8) {
-#if defined(GLA_ZT_E_F) && defined(GLA_ZT_PX_F)
-        (void)n;
-        return {GLA_ZT_E_F, GLA_ZT_PX_F};
-#else
         switch (n) {
             // BEGIN TUNED C64
             case 16: return {4, 16};
@@ -48,12 +60,7 @@ constexpr ZShape ztune(int complex_bytes, int n)
             case 145: return {5, 8};
             default: return {8, 16};
         }
-#endif
     }
-#if defined(GLA_ZT_E_D) && defined(GLA_ZT_PX_D)
-    (void)n;
-    return {GLA_ZT_E_D, GLA_ZT_PX_D};
-#else
     switch (n) {
         // BEGIN TUNED C128
             case 20: return {2, 8};
@@ -66,6 +73,85 @@ constexpr ZShape ztune(int complex_bytes, int n)
             case 512: return {8, 4};
         // END TUNED C128
         default: return {4, 8};
+    }
+}
+
+// small cross-sections; unlisted lengths take the large-grid shape
+constexpr ZShape ztune_small(int complex_bytes, int n)
+{
+    if (complex_bytes == 8) {
+        switch (n) {
+            // BEGIN TUNED C64 SMALL
+            // END TUNED C64 SMALL
+            default: return ztune_large(complex_bytes, n);
+        }
+    }
+    switch (n) {
+        // BEGIN TUNED C128 SMALL
+        // END TUNED C128 SMALL
+        default: return ztune_large(complex_bytes, n);
+    }
+}
+
+constexpr ZShape ztune(int complex_bytes, int n, bool small)
+{
+#if defined(GLA_ZT_E_F) && defined(GLA_ZT_PX_F)
+    if (complex_bytes == 8) return {GLA_ZT_E_F, GLA_ZT_PX_F};
+#endif
+#if defined(GLA_ZT_E_D) && defined(GLA_ZT_PX_D)
+    if (complex_bytes == 16) return {GLA_ZT_E_D, GLA_ZT_PX_D};
+#endif
+    return small ? ztune_small(complex_bytes, n) : ztune_large(complex_bytes, n);
+}
+
+constexpr RShape rtune(int complex_bytes, int n)
+{
+    if (complex_bytes == 8) {
+#if defined(GLA_RT_EX_F) && defined(GLA_RT_EY_F) && defined(GLA_RT_PXY_F)
+        (void)n;
+        return {GLA_RT_EX_F, GLA_RT_EY_F, GLA_RT_PXY_F};
+#else
+        switch (n) {
+            // BEGIN TUNED ROWS C64
+            // END TUNED ROWS C64
+            default: return n <= 160 ? RShape{4, 4, 16} : RShape{8, 8, 16};
+        }
+#endif
+    }
+#if defined(GLA_RT_EX_D) && defined(GLA_RT_EY_D) && defined(GLA_RT_PXY_D)
+    (void)n;
+    return {GLA_RT_EX_D, GLA_RT_EY_D, GLA_RT_PXY_D};
+#else
+    switch (n) {
+        // BEGIN TUNED ROWS C128
+        // END TUNED ROWS C128
+        default: return n <= 128 ? RShape{4, 4, 8} : RShape{8, 8, 8};
+    }
+#endif
+}
+
+constexpr int ptune(int complex_bytes, int n)
+{
+    if (complex_bytes == 8) {
+#if defined(GLA_PT_F)
+        (void)n;
+        return GLA_PT_F;
+#else
+        switch (n) {
+            // BEGIN TUNED PLAN C64
+            // END TUNED PLAN C64
+            default: return 0;
+        }
+#endif
+    }
+#if defined(GLA_PT_D)
+    (void)n;
+    return GLA_PT_D;
+#else
+    switch (n) {
+        // BEGIN TUNED PLAN C128
+        // END TUNED PLAN C128
+        default: return 0;
     }
 #endif
 }
