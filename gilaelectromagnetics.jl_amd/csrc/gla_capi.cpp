@@ -211,13 +211,20 @@ int64_t int_ratio(int64_t an, int64_t ad, int64_t bn, int64_t bd)
     return q > (__int128)(1 << 20) ? 0 : (int64_t)q;
 }
 
-// exact: the layout of Fourier data handed in (egoFur, files): B itself when
-// it has a compiled transform.  Otherwise (operators built from geometry) a B
-// without a fast transform (2^a x {1, 3, 5}) is embedded into the next fast
-// one -- the circulant only needs L >= S + T - 1, and the lengths with a
-// factor beyond 5 (29, 81, 145) transform several times more slowly.
+// Device branch lengths by layout:
+//   DIMS_FAST  operators built from geometry: a B without a fast transform
+//              (2^a x {1, 3, 5}) is embedded into the next fast one -- the
+//              circulant only needs L >= S + T - 1, and the lengths with a
+//              factor beyond 5 (29, 81, 145) transform several times more
+//              slowly;
+//   DIMS_EXACT Fourier data handed in: B itself when it has a compiled
+//              transform, else the next fast length (the data is then
+//              re-embedded, gla::refur);
+//   DIMS_REF   the reference's own layout of that data: B on every axis.
 // trg / src are the volumes after regeneration (even_pair).
-gla_status dims_impl(const gla_vol* trg, const gla_vol* src, bool exact, gla_dims* out)
+enum { DIMS_FAST = 0, DIMS_EXACT = 1, DIMS_REF = 2 };
+
+gla_status dims_impl(const gla_vol* trg, const gla_vol* src, int mode, gla_dims* out)
 {
     if (!trg || !out) return fail(GLA_ERR_ARG, "null argument");
     gla_status s = check_vol(*trg, "target volume");
@@ -255,7 +262,7 @@ gla_status dims_impl(const gla_vol* trg, const gla_vol* src, bool exact, gla_dim
         d.brn[a] = (d.trg_cel[a] + d.src_cel[a]) / 2;
         // an embedded axis stores its full branches
         const int b = (int)d.brn[a];
-        const bool keep = exact ? gla::len_supported(b) : gla::len_fast(b);
+        const bool keep = mode == DIMS_REF || (mode == DIMS_EXACT ? gla::len_supported(b) : gla::len_fast(b));
         d.dev_brn[a] = keep ? d.brn[a] : gla::len_next_supported(b);
         if (d.dev_brn[a] == 0)
             return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) + " exceeds the compiled transforms");
@@ -291,7 +298,14 @@ gla_status gla_opr_dims_for(const gla_vol* trg, const gla_vol* src, gla_dims* ou
     if (!trg || !out) return fail(GLA_ERR_ARG, "null argument");
     gla_vol t2, s2;
     even_pair(trg, src, t2, s2);
-    return dims_impl(&t2, src ? &s2 : nullptr, true, out);
+    // the layout the caller hands in, with the lengths the GPU will use
+    gla_dims ref, dev;
+    gla_status s = dims_impl(&t2, src ? &s2 : nullptr, DIMS_REF, &ref);
+    if (s) return s;
+    if ((s = dims_impl(&t2, src ? &s2 : nullptr, DIMS_EXACT, &dev))) return s;
+    for (int a = 0; a < 3; ++a) ref.dev_brn[a] = dev.dev_brn[a];
+    *out = ref;
+    return GLA_OK;
 }
 
 gla_status gla_vol_even_gen(const gla_vol* in, gla_vol* out)
@@ -535,14 +549,32 @@ gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, con
     gla_vol t2, s2;
     even_pair(trg, src, t2, s2);
     const gla_vol* sp = src ? &s2 : nullptr;
-    gla_dims d;
-    if ((s = dims_impl(&t2, sp, true, &d))) return s;
-    for (int a = 0; a < 3; ++a)
-        if (d.dev_brn[a] != d.brn[a])
-            return fail(GLA_ERR_UNSUPPORTED, "branch length " + std::to_string(d.brn[a]) +
-                                                 " has no compiled transform; Fourier data in the reference layout "
-                                                 "cannot be re-embedded (build the operator with gla_opr_create)");
-    return create_common(opts, &t2, sp, d, fur, fur_dtype, fur_len, out);
+    gla_dims ref, d;
+    if ((s = dims_impl(&t2, sp, DIMS_REF, &ref))) return s;
+    if ((s = dims_impl(&t2, sp, DIMS_EXACT, &d))) return s;
+    bool same = true;
+    for (int a = 0; a < 3; ++a) same = same && d.dev_brn[a] == ref.brn[a];
+    if (same) return create_common(opts, &t2, sp, d, fur, fur_dtype, fur_len, out);
+    // a branch length without a compiled transform: the data is re-expressed
+    // at the next fast length (gla::refur) in FP64
+    const int64_t Ut = ref.fur_ext[0] * ref.fur_ext[1] * ref.fur_ext[2];
+    if (fur_len != fur_count(ref))
+        return fail(GLA_ERR_SIZE, "Fourier Green data has " + std::to_string(fur_len) + " elements, expected 8*" +
+                                      std::to_string(Ut) + "*6*" + std::to_string(n_src_part(ref) * n_trg_part(ref)));
+    const bool ok = fur_dtype == GLA_C64 ? finite_all((const float*)fur, fur_len)
+                                         : finite_all((const double*)fur, fur_len);
+    if (!ok) return fail(GLA_ERR_NONFINITE, "Fourier information contains non-numeric values.");
+    std::vector<std::complex<double>> furd((size_t)fur_len), re;
+    for (int64_t i = 0; i < fur_len; ++i)
+        furd[(size_t)i] = fur_dtype == GLA_C64
+                              ? std::complex<double>(((const float*)fur)[2 * i], ((const float*)fur)[2 * i + 1])
+                              : std::complex<double>(((const double*)fur)[2 * i], ((const double*)fur)[2 * i + 1]);
+    std::string err;
+    {
+        DevGuard g(opts->device);
+        if ((s = gla::refur(ref, d, furd, re, err))) return fail(s, err);
+    }
+    return create_common(opts, &t2, sp, d, re.data(), GLA_C128, (int64_t)re.size(), out);
 }
 
 }  // extern "C"
@@ -617,7 +649,7 @@ gla_status gla_opr_create(const gla_opts* opts, const gla_vol* trg, const gla_vo
     even_pair(trg, src, t2, s2);
     const gla_vol* sp = src ? &s2 : nullptr;
     gla_dims d;
-    if ((s = dims_impl(&t2, sp, false, &d))) return s;
+    if ((s = dims_impl(&t2, sp, DIMS_FAST, &d))) return s;
     std::vector<std::complex<double>> fur;
     std::string err;
     s = gla::setup_fur(*opts, t2, sp ? s2 : t2, d, fur, err);
@@ -763,7 +795,8 @@ gla_status gla_opr_load(const char* path, int32_t dtype, int32_t device, gla_opr
     // an operator built from geometry
     bool match = false;
     for (int exact = 0; exact < 2 && !match; ++exact) {
-        if (dims_impl(&hd.trg, sp, exact != 0, &d) != GLA_OK) return bad("volumes in the header are inconsistent");
+        if (dims_impl(&hd.trg, sp, exact != 0 ? DIMS_EXACT : DIMS_FAST, &d) != GLA_OK)
+            return bad("volumes in the header are inconsistent");
         match = true;
         for (int a = 0; a < 3; ++a)
             if (d.brn[a] != hd.brn[a] || d.dev_brn[a] != hd.dev_brn[a] || d.fur_ext[a] != hd.fur_ext[a] ||

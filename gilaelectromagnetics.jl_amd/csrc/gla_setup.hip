@@ -661,6 +661,145 @@ struct DevBuf {
     ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
+// ---- Fourier data at branch lengths without a compiled transform ----
+
+// the full spectrum (L1,L2,L3, 6) of partition pair `pair` of P from the
+// stored branches in the reference layout (the mirror rules of mulBrnHst!,
+// src/glaActSup.jl:691-753, on symmetric axes; frequency k = 2j + p)
+__global__ void k_expand(double2* __restrict__ F, const double2* __restrict__ fur, AxisMap a1, AxisMap a2,
+                         AxisMap a3, int pair, int P)
+{
+    const int L1 = 2 * a1.B, L2 = 2 * a2.B, L3 = 2 * a3.B;
+    const int64_t Lt = (int64_t)L1 * L2 * L3;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Lt * 6) return;
+    const int64_t q = p % Lt;
+    const int c = (int)(p / Lt);
+    const int k1 = (int)(q % L1), k2 = (int)((q / L1) % L2), k3 = (int)(q / ((int64_t)L1 * L2));
+    const int p1 = k1 & 1, p2 = k2 & 1, p3 = k3 & 1;
+    int j1, j2, j3, s1, s2, s3;
+    axis_lookup(a1, p1, k1 >> 1, j1, s1);
+    axis_lookup(a2, p2, k2 >> 1, j2, s2);
+    axis_lookup(a3, p3, k3 >> 1, j3, s3);
+    const int b = (p1 << 2) | (p2 << 1) | p3;
+    const int64_t Ut = (int64_t)a1.U * a2.U * a3.U;
+    const double2 v = fur[(((int64_t)b * P + pair) * 6 + c) * Ut + ((int64_t)j3 * a2.U + j2) * a1.U + j1];
+    const double sg = c == 3 ? s1 * s2 : (c == 4 ? s1 * s3 : (c == 5 ? s2 * s3 : 1));
+    F[p] = make_double2(sg * v.x, sg * v.y);
+}
+
+// one axis of an inverse DFT of any length by direct summation in FP64:
+// out[.. m ..] = sum_k in[.. k ..] e^{2 pi i k m / L}, the phase read from the
+// table of the L roots at the exact index k m mod L; layout (L1,L2,L3, 6)
+__global__ void k_idft_axis(double2* __restrict__ out, const double2* __restrict__ in, int L1, int L2, int L3,
+                            int axis, const double2* __restrict__ roots)
+{
+    const int64_t Lt = (int64_t)L1 * L2 * L3;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Lt * 6) return;
+    const int64_t q = p % Lt;
+    const int n1 = (int)(q % L1), n2 = (int)((q / L1) % L2);
+    const int m = axis == 0 ? n1 : (axis == 1 ? n2 : (int)(q / ((int64_t)L1 * L2)));
+    const int L = axis == 0 ? L1 : (axis == 1 ? L2 : L3);
+    const int64_t st = axis == 0 ? 1 : (axis == 1 ? L1 : (int64_t)L1 * L2);
+    const double2* src = in + (p - (int64_t)m * st);
+    double ar = 0, ai = 0;
+    int idx = 0;
+    for (int k = 0; k < L; ++k) {
+        const double2 v = src[(int64_t)k * st];
+        const double2 w = roots[idx];
+        ar += v.x * w.x - v.y * w.y;
+        ai += v.x * w.y + v.y * w.x;
+        idx += m;
+        if (idx >= L) idx -= L;
+    }
+    out[p] = make_double2(ar, ai);
+}
+
+// the circulant of an exact grid L = S + T re-embedded on a grid L' >= L:
+// separations 0..T-1 at n' = n, -(S-1)..-1 at the top of L'; zero elsewhere;
+// scaled by `scale` (the inverse transform's 1 / (L1 L2 L3))
+__global__ void k_reembed(double2* __restrict__ out, const double2* __restrict__ in, int L1, int L2, int L3,
+                          int M1, int M2, int M3, int T1, int T2, int T3, int S1, int S2, int S3, double scale)
+{
+    const int64_t Mt = (int64_t)M1 * M2 * M3, Lt = (int64_t)L1 * L2 * L3;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Mt * 6) return;
+    const int64_t q = p % Mt;
+    const int c = (int)(p / Mt);
+    const int m1 = (int)(q % M1), m2 = (int)((q / M1) % M2), m3 = (int)(q / ((int64_t)M1 * M2));
+    auto src = [](int m, int M, int L, int T, int S) { return m < T ? m : (m > M - S ? m - M + L : -1); };
+    const int n1 = src(m1, M1, L1, T1, S1), n2 = src(m2, M2, L2, T2, S2), n3 = src(m3, M3, L3, T3, S3);
+    if (n1 < 0 || n2 < 0 || n3 < 0) {
+        out[p] = make_double2(0, 0);
+        return;
+    }
+    const double2 v = in[(int64_t)c * Lt + n1 + (int64_t)L1 * (n2 + (int64_t)L2 * n3)];
+    out[p] = make_double2(scale * v.x, scale * v.y);
+}
+
+// The forward transform of circulants (six components on the L = 2B grid,
+// FP64, layout (L1,L2,L3,6)) through the pass kernels with S = T = L on every
+// axis, and the branch extraction of k_gather
+struct FwdGrid {
+    int B[3] = {0, 0, 0};
+    int64_t Lt = 0;
+    Geom g{};
+    DevTab tb{};
+    DevBuf dtab, dw1, dw2;
+};
+
+hipError_t fwd_grid_init(FwdGrid& f, const int B[3])
+{
+    int L[3];
+    for (int a = 0; a < 3; ++a) {
+        f.B[a] = B[a];
+        L[a] = 2 * B[a];
+    }
+    f.Lt = (int64_t)L[0] * L[1] * L[2];
+    Geom& g = f.g;
+    for (int a = 0; a < 3; ++a) { g.S[a] = L[a]; g.T[a] = L[a]; g.B[a] = B[a]; g.L[a] = L[a]; }
+    g.Y1 = L[1];
+    g.Z = L[2];
+    g.xs = f.Lt * 3;
+    g.ys = f.Lt * 3;
+    g.w1s = f.Lt * 3;
+    g.w2s = f.Lt * 3;
+    g.xp = whole_part(g.S);
+    g.yp = whole_part(g.T);
+    std::vector<double2> tab;
+    int64_t off[3][2];
+    for (int a = 0; a < 3; ++a) {
+        off[a][0] = (int64_t)tab.size();
+        for (int n = 0; n < B[a]; ++n) tab.push_back(make_double2(std::cos(-M_PI * n / B[a]), std::sin(-M_PI * n / B[a])));
+        off[a][1] = (int64_t)tab.size();
+        for (int n = 0; n < B[a]; ++n)
+            tab.push_back(make_double2(std::cos(-2 * M_PI * n / B[a]), std::sin(-2 * M_PI * n / B[a])));
+    }
+    hipError_t e = hipMalloc(&f.dtab.p, tab.size() * sizeof(double2));
+    if (e == hipSuccess) e = hipMemcpy(f.dtab.p, tab.data(), tab.size() * sizeof(double2), hipMemcpyHostToDevice);
+    for (int a = 0; a < 3; ++a) {
+        f.tb.phz[a] = (const double2*)f.dtab.p + off[a][0];
+        f.tb.tw[a] = (const double2*)f.dtab.p + off[a][1];
+    }
+    if (e == hipSuccess) e = hipMalloc(&f.dw1.p, (size_t)f.Lt * 6 * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc(&f.dw2.p, (size_t)f.Lt * 6 * sizeof(double2));
+    return e;
+}
+
+// transform circulant crc and extract its branches into block `pair` of P of
+// dfur (stored extents d.fur_ext)
+hipError_t fwd_grid_run(FwdGrid& f, const void* crc, double2* dfur, const gla_dims& d, int pair, int P)
+{
+    hipError_t e = run_fwd3<double>(f.g, f.tb, crc, f.dw1.p, f.dw2.p, 2, nullptr);
+    if (e != hipSuccess) return e;
+    const int64_t Ut = d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2];
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((Ut * 48 + 255) / 256)), dim3(256), 0, nullptr, dfur,
+                       (const double2*)f.dw2.p, f.B[0], f.B[1], f.B[2], (int)d.fur_ext[0], (int)d.fur_ext[1],
+                       (int)d.fur_ext[2], pair, P);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& src, const gla_dims& d,
@@ -757,41 +896,14 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
         const gla_status st = singular_values(G.scl, G.zr, G.zi, opts.int_ord > 0 ? opts.int_ord : 32, w18, err);
         if (st) return st;
     }
-    DevBuf dgl, dcrc, dw1, dw2, dtab, dtoe, dfur, dcnt, dw18;
+    DevBuf dgl, dcrc, dtoe, dfur, dcnt, dw18;
     SETUP_HIP(hipMalloc(&dgl.p, gl.size() * sizeof(double)));
     SETUP_HIP(hipMemcpy(dgl.p, gl.data(), gl.size() * sizeof(double), hipMemcpyHostToDevice));
     SETUP_HIP(hipMalloc(&dw18.p, sizeof(w18)));
     SETUP_HIP(hipMemcpy(dw18.p, w18, sizeof(w18), hipMemcpyHostToDevice));
     SETUP_HIP(hipMalloc(&dcrc.p, (size_t)Lt * 6 * sizeof(double2)));
-    // forward transform over the full L grid: S = L on every axis
-    Geom g{};
-    for (int a = 0; a < 3; ++a) { g.S[a] = L[a]; g.T[a] = L[a]; g.B[a] = B[a]; g.L[a] = L[a]; }
-    g.Y1 = L[1];
-    g.Z = L[2];
-    g.xs = Lt * 3;
-    g.ys = Lt * 3;
-    g.w1s = Lt * 3;
-    g.w2s = Lt * 3;
-    g.xp = whole_part(g.S);
-    g.yp = whole_part(g.T);
-    std::vector<double2> tab;
-    int64_t off[3][2];
-    for (int a = 0; a < 3; ++a) {
-        off[a][0] = (int64_t)tab.size();
-        for (int n = 0; n < B[a]; ++n) tab.push_back(make_double2(std::cos(-M_PI * n / B[a]), std::sin(-M_PI * n / B[a])));
-        off[a][1] = (int64_t)tab.size();
-        for (int n = 0; n < B[a]; ++n)
-            tab.push_back(make_double2(std::cos(-2 * M_PI * n / B[a]), std::sin(-2 * M_PI * n / B[a])));
-    }
-    SETUP_HIP(hipMalloc(&dtab.p, tab.size() * sizeof(double2)));
-    SETUP_HIP(hipMemcpy(dtab.p, tab.data(), tab.size() * sizeof(double2), hipMemcpyHostToDevice));
-    DevTab tb{};
-    for (int a = 0; a < 3; ++a) {
-        tb.phz[a] = (const double2*)dtab.p + off[a][0];
-        tb.tw[a] = (const double2*)dtab.p + off[a][1];
-    }
-    SETUP_HIP(hipMalloc(&dw1.p, (size_t)Lt * 6 * sizeof(double2)));
-    SETUP_HIP(hipMalloc(&dw2.p, (size_t)Lt * 6 * sizeof(double2)));
+    FwdGrid fg;
+    SETUP_HIP(fwd_grid_init(fg, B));
     const int64_t Ut = d.fur_ext[0] * d.fur_ext[1] * d.fur_ext[2];
     const int P = Ps * Pt;
     SETUP_HIP(hipMalloc(&dfur.p, (size_t)Ut * 48 * P * sizeof(double2)));
@@ -848,11 +960,7 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
         hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)((Lt * 6 + 255) / 256)), dim3(256), 0, nullptr,
                            (const double2*)dcrc.p, Lt * 6, (int*)dcnt.p);
         SETUP_HIP(hipGetLastError());
-        SETUP_HIP(run_fwd3<double>(g, tb, dcrc.p, dw1.p, dw2.p, 2, nullptr));
-        hipLaunchKernelGGL(k_gather, dim3((unsigned)((Ut * 48 + 255) / 256)), dim3(256), 0, nullptr,
-                           (double2*)dfur.p, (const double2*)dw2.p, B[0], B[1], B[2], (int)d.fur_ext[0],
-                           (int)d.fur_ext[1], (int)d.fur_ext[2], pair, P);
-        SETUP_HIP(hipGetLastError());
+        SETUP_HIP(fwd_grid_run(fg, dcrc.p, (double2*)dfur.p, d, pair, P));
     }
     int bad = 0;
     SETUP_HIP(hipMemcpy(&bad, dcnt.p, sizeof(int), hipMemcpyDeviceToHost));
@@ -862,6 +970,83 @@ gla_status setup_fur(const gla_opts& opts, const gla_vol& trg, const gla_vol& sr
     }
     fur.resize((size_t)(Ut * 48 * P));
     SETUP_HIP(hipMemcpy(fur.data(), dfur.p, fur.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    return GLA_OK;
+#undef SETUP_HIP
+}
+
+gla_status refur(const gla_dims& dx, const gla_dims& df, const std::vector<std::complex<double>>& in,
+                 std::vector<std::complex<double>>& out, std::string& err)
+{
+#define SETUP_HIP(call)                                                             \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return GLA_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+    const int P = (int)(dx.src_div[0] * dx.src_div[1] * dx.src_div[2] * dx.trg_div[0] * dx.trg_div[1] *
+                        dx.trg_div[2]);
+    int L[3], M[3], Bf[3];
+    AxisMap am[3];
+    for (int a = 0; a < 3; ++a) {
+        L[a] = 2 * (int)dx.brn[a];
+        Bf[a] = (int)df.dev_brn[a];
+        M[a] = 2 * Bf[a];
+        am[a].B = (int)dx.brn[a];
+        am[a].U = (int)dx.fur_ext[a];
+        am[a].sym = (dx.sym[a] && dx.fur_ext[a] != dx.brn[a]) ? 1 : 0;
+        const int T = (int)dx.trg_cel[a];
+        am[a].fe = (T + 1) / 2 + (T % 2 == 0 ? 1 : 0);
+        am[a].fo = (T + 1) / 2;
+        if (!len_supported(Bf[a])) {
+            err = "branch length " + std::to_string(Bf[a]) + " has no compiled transform";
+            return GLA_ERR_UNSUPPORTED;
+        }
+    }
+    const int64_t Lt = (int64_t)L[0] * L[1] * L[2], Mt = (int64_t)M[0] * M[1] * M[2];
+    const int64_t Uin = dx.fur_ext[0] * dx.fur_ext[1] * dx.fur_ext[2];
+    const int64_t Uout = df.fur_ext[0] * df.fur_ext[1] * df.fur_ext[2];
+    DevBuf din, dA, dB, droots, dcrc, dfur;
+    SETUP_HIP(hipMalloc(&din.p, (size_t)Uin * 48 * P * sizeof(double2)));
+    SETUP_HIP(hipMemcpy(din.p, in.data(), (size_t)Uin * 48 * P * sizeof(double2), hipMemcpyHostToDevice));
+    SETUP_HIP(hipMalloc(&dA.p, (size_t)Lt * 6 * sizeof(double2)));
+    SETUP_HIP(hipMalloc(&dB.p, (size_t)Lt * 6 * sizeof(double2)));
+    SETUP_HIP(hipMalloc(&dcrc.p, (size_t)Mt * 6 * sizeof(double2)));
+    SETUP_HIP(hipMalloc(&dfur.p, (size_t)Uout * 48 * P * sizeof(double2)));
+    // roots e^{2 pi i r / L} of each axis, back to back
+    std::vector<double2> roots;
+    int roff[3];
+    for (int a = 0; a < 3; ++a) {
+        roff[a] = (int)roots.size();
+        for (int r = 0; r < L[a]; ++r)
+            roots.push_back(make_double2(std::cos(2 * M_PI * r / L[a]), std::sin(2 * M_PI * r / L[a])));
+    }
+    SETUP_HIP(hipMalloc(&droots.p, roots.size() * sizeof(double2)));
+    SETUP_HIP(hipMemcpy(droots.p, roots.data(), roots.size() * sizeof(double2), hipMemcpyHostToDevice));
+    FwdGrid fg;
+    SETUP_HIP(fwd_grid_init(fg, Bf));
+    const unsigned gl = (unsigned)((Lt * 6 + 255) / 256), gm = (unsigned)((Mt * 6 + 255) / 256);
+    for (int pair = 0; pair < P; ++pair) {
+        hipLaunchKernelGGL(k_expand, dim3(gl), dim3(256), 0, nullptr, (double2*)dA.p, (const double2*)din.p, am[0],
+                           am[1], am[2], pair, P);
+        SETUP_HIP(hipGetLastError());
+        void* src = dA.p;
+        void* dst = dB.p;
+        for (int a = 0; a < 3; ++a) {
+            hipLaunchKernelGGL(k_idft_axis, dim3(gl), dim3(256), 0, nullptr, (double2*)dst, (const double2*)src, L[0],
+                               L[1], L[2], a, (const double2*)droots.p + roff[a]);
+            SETUP_HIP(hipGetLastError());
+            std::swap(src, dst);
+        }
+        hipLaunchKernelGGL(k_reembed, dim3(gm), dim3(256), 0, nullptr, (double2*)dcrc.p, (const double2*)src, L[0],
+                           L[1], L[2], M[0], M[1], M[2], (int)dx.trg_cel[0], (int)dx.trg_cel[1], (int)dx.trg_cel[2],
+                           (int)dx.src_cel[0], (int)dx.src_cel[1], (int)dx.src_cel[2], 1.0 / (double)Lt);
+        SETUP_HIP(hipGetLastError());
+        SETUP_HIP(fwd_grid_run(fg, dcrc.p, (double2*)dfur.p, df, pair, P));
+    }
+    out.resize((size_t)(Uout * 48 * P));
+    SETUP_HIP(hipMemcpy(out.data(), dfur.p, out.size() * sizeof(double2), hipMemcpyDeviceToHost));
     return GLA_OK;
 #undef SETUP_HIP
 }

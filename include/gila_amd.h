@@ -99,7 +99,8 @@ const char* gla_last_error(void);
 gla_status gla_init(int32_t device);
 
 /* Geometry that gla_opr_create_from_fur expects for (trg, src): the reference
- * derives it in GlaOprMem (src/glaMemSup.jl:16-156).  src == NULL means a
+ * derives it in GlaOprMem (src/glaMemSup.jl:16-156); dev_brn gives the lengths
+ * the GPU will transform.  src == NULL means a
  * self operator.  An external pair whose cell counts sum to an odd number on
  * some axis is first regenerated with gla_vol_even_gen, as GlaOprMem does
  * (src/glaMemSup.jl:34-40).  Cell scales that differ by an integer factor
@@ -120,7 +121,12 @@ gla_status gla_vol_even_gen(const gla_vol* in, gla_vol* out);
  * [xx,yy,zz,xy,xz,yz] and Ps / Pt the source / target partition counts
  * (1 and 1 for equal cell scales); fur_dtype gives its element type; fur_len
  * is the number of complex elements (8*U1*U2*U3*6*Ps*Pt).  Non-finite values
- * fail with GLA_ERR_NONFINITE as the reference's check does. */
+ * fail with GLA_ERR_NONFINITE as the reference's check does.  Any branch
+ * length B <= 512 is accepted: data at a length without a compiled transform
+ * is re-expressed in FP64 at the next fast length (the circulant it encodes,
+ * recovered by an inverse transform, is re-embedded on the longer grid and
+ * transformed there), so gla_opr_get_dims then reports that length in
+ * dev_brn and gla_opr_export_fur returns the data at it. */
 gla_status gla_opr_create_from_fur(const gla_opts* opts, const gla_vol* trg, const gla_vol* src,
                                    const void* fur, int32_t fur_dtype, int64_t fur_len,
                                    gla_opr_t* out);
