@@ -22,7 +22,7 @@
 
 namespace gla {
 
-constexpr int ZSMALL_COLS = 128 * 128;
+constexpr int ZSMALL_COLS = 2 * 128 * 128;
 
 struct ZShape {
     int e, px;
@@ -38,15 +38,15 @@ constexpr ZShape ztune_large(int complex_bytes, int n)
         switch (n) {
             // BEGIN TUNED C64
             case 16: return {4, 16};
-            case 20: return {4, 8};
             case 24: return {4, 16};
             case 32: return {4, 8};
             case 40: return {4, 16};
             case 48: return {4, 16};
+            case 64: return {4, 8};
             case 80: return {4, 8};
             case 96: return {4, 8};
             case 128: return {4, 8};
-            case 160: return {4, 16};
+            case 160: return {4, 4};
             case 192: return {8, 8};
             case 256: return {8, 8};
             case 320: return {4, 8};
@@ -63,8 +63,7 @@ constexpr ZShape ztune_large(int complex_bytes, int n)
     }
     switch (n) {
         // BEGIN TUNED C128
-            case 20: return {2, 8};
-            case 64: return {4, 4};
+            case 16: return {4, 16};
             case 160: return {4, 4};
             case 192: return {4, 4};
             case 256: return {8, 4};
@@ -82,12 +81,19 @@ constexpr ZShape ztune_small(int complex_bytes, int n)
     if (complex_bytes == 8) {
         switch (n) {
             // BEGIN TUNED C64 SMALL
+            case 20: return {4, 16};
+            case 64: return {8, 8};
+            case 192: return {4, 4};
             // END TUNED C64 SMALL
             default: return ztune_large(complex_bytes, n);
         }
     }
     switch (n) {
         // BEGIN TUNED C128 SMALL
+            case 20: return {4, 16};
+            case 40: return {4, 16};
+            case 48: return {4, 16};
+            case 64: return {4, 4};
         // END TUNED C128 SMALL
         default: return ztune_large(complex_bytes, n);
     }
@@ -113,6 +119,8 @@ constexpr RShape rtune(int complex_bytes, int n)
 #else
         switch (n) {
             // BEGIN TUNED ROWS C64
+            case 48: return RShape{8, 4, 16};
+            case 160: return RShape{8, 4, 16};
             // END TUNED ROWS C64
             default: return n <= 160 ? RShape{4, 4, 16} : RShape{8, 8, 16};
         }
@@ -124,6 +132,7 @@ constexpr RShape rtune(int complex_bytes, int n)
 #else
     switch (n) {
         // BEGIN TUNED ROWS C128
+            case 48: return RShape{8, 4, 8};
         // END TUNED ROWS C128
         default: return n <= 128 ? RShape{4, 4, 8} : RShape{8, 8, 8};
     }

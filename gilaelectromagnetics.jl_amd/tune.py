@@ -10,22 +10,22 @@ length (compile-time overrides of gla_tune.hpp):
 
   zpass  pass 3: (elements per thread, columns per workgroup), timed on a
          large cross-section (128 x 128 x N: 256 x 256 columns) and a small
-         one (32 x 32 x N: 64 x 64 columns, below ZSMALL_COLS) -- the two grid
-         classes of ztune
+         one (64 x 64 x N: 128 x 128 columns, below ZSMALL_COLS) -- the two
+         grid classes of ztune
   rows   passes 1, 2, 4, 5: elements per thread of the axis-1 rows and the
-         axis-2 columns and the axis-2 bundle width, timed on N x N x 32 (both
-         row axes of length N)
+         axis-2 columns and the axis-2 bundle width, timed on N^3 (N <= 128;
+         N x N x 128 beyond: both row axes of length N)
   plan   the radix plan policy (every thread busy in every stage, against the
          fewest LDS exchanges) for all passes of a length, timed as pass 3 on
-         the large cross-section plus the row passes on N x N x 32
+         the large cross-section plus the row passes on the rows grid
 
 Operators come from all-zero Green data (the timing does not depend on the
 values), so no setup runs.  Each (part, element type, length[, class]) keeps
-the fastest candidate when it beats the current default by more than
---margin (5%): a timing grid is not every workload, small differences stay
-with the default.  (A 64 x 64 x N grid for every length understated the pass-3
-gain of a shape at 96^3 threefold against the cube: the cross-section
-matters, hence the classes.)
+its current table entry (or default) unless a candidate beats it by more than
+--margin (5%): a timing grid is not every workload, and differences within
+the run-to-run noise stay with the incumbent.  (A 64 x 64 x N grid for every
+length understated the pass-3 gain of a shape at 96^3 threefold against the
+cube: the cross-section matters, hence the classes.)
 """
 import argparse
 import json
@@ -117,9 +117,9 @@ def measure_one(name, lengths):
             if part in ("zpass", "plan"):
                 rec["large_us"] = timed((128, 128, n), typ)
             if part == "zpass":
-                rec["small_us"] = timed((32, 32, n), typ)
+                rec["small_us"] = timed((64, 64, n), typ)
             if part in ("rows", "plan"):
-                rec["rows_us"] = timed((n, n, 32), typ)
+                rec["rows_us"] = timed((n, n, min(n, 128)), typ)
             print(json.dumps(rec), flush=True)
 
 
@@ -140,12 +140,24 @@ def _sub(src, tag, rows):
                   lambda m: m.group(1) + body + m.group(3), src, flags=re.S)
 
 
+def _current(src, tag, pat):
+    """Entries of the table between the markers of `tag` in gla_tune.hpp."""
+    m = re.search(rf"// BEGIN TUNED {tag}\n(.*?)// END TUNED {tag}", src, flags=re.S)
+    out = {}
+    for mm in re.finditer(pat, m.group(1) if m else ""):
+        out[int(mm.group(1))] = tuple(int(v) for v in mm.groups()[1:])
+    return out
+
+
 def table(results, margin):
     recs = [json.loads(line) for line in open(results) if line.strip()]
     lines = []
     src = open(os.path.join(CSRC, "gla_tune.hpp")).read()
+    zpat = r"case (\d+): return \{(\d+), (\d+)\};"
+    rpat = r"case (\d+): return RShape\{(\d+), (\d+), (\d+)\};"
+    ppat = r"case (\d+): return (\d+);"
     for dt, tag in (("c64", "C64"), ("c128", "C128")):
-        # pass 3, large class: fastest shape where it beats the fallback default
+        # pass 3, large class: the fastest shape where it beats the current entry
         zl, zs = {}, {}
         for r in recs:
             if r["part"] != "zpass" or r["dtype"] != dt:
@@ -153,11 +165,12 @@ def table(results, margin):
             zl.setdefault(r["n"], {})[tuple(r["choice"])] = r["large_us"][2]
             zs.setdefault(r["n"], {})[tuple(r["choice"])] = r["small_us"][2]
         large, small = {}, {}
+        cur = _current(src, tag, zpat)
         for n, by in zl.items():
-            d = ZDEFAULT[dt]
+            d = cur.get(n, ZDEFAULT[dt])
             best = min(by, key=by.get)
             large[n] = best if d not in by or by[best] < (1 - margin) * by[d] else d
-            lines.append(f"{dt} zpass large {n}: {large[n]} {by[large[n]]:.1f} us (default {d} {by.get(d, 0):.1f})")
+            lines.append(f"{dt} zpass large {n}: {large[n]} {by[large[n]]:.1f} us (current {d} {by.get(d, 0):.1f})")
         for n, by in zs.items():
             ref = large[n]
             best = min(by, key=by.get)
@@ -178,16 +191,17 @@ def table(results, margin):
             rx.setdefault(r["n"], {})[ex] = min(t[0] + t[4], rx.get(r["n"], {}).get(ex, 1e30))
             ry.setdefault(r["n"], {})[(ey, pxy)] = t[1] + t[3]
         rows = []
+        rcur = _current(src, "ROWS " + tag, rpat)
         for n in sorted(rx):
-            d = rdefault(dt, n)
+            d = rcur.get(n, rdefault(dt, n))
             bx = min(rx[n], key=rx[n].get)
             ex = bx if d[0] not in rx[n] or rx[n][bx] < (1 - margin) * rx[n][d[0]] else d[0]
             by_ = min(ry[n], key=ry[n].get)
             dy = (d[1], d[2])
             ey = by_ if dy not in ry[n] or ry[n][by_] < (1 - margin) * ry[n][dy] else dy
             lines.append(f"{dt} rows {n}: ex {ex} ({rx[n][ex]:.1f} us), ey/pxy {ey} ({ry[n][ey]:.1f} us); "
-                         f"default {d}")
-            if (ex, ey[0], ey[1]) != d:
+                         f"current {d}")
+            if (ex, ey[0], ey[1]) != rdefault(dt, n):
                 rows.append(f"            case {n}: return RShape{{{ex}, {ey[0]}, {ey[1]}}};")
         src = _sub(src, "ROWS " + tag, rows)
         # plan policy: pass 3 on the large grid plus the row passes
@@ -197,9 +211,12 @@ def table(results, margin):
                 continue
             pol.setdefault(r["n"], {})[r["choice"]] = r["large_us"][2] + sum(r["rows_us"][i] for i in (0, 1, 3, 4))
         prow = []
+        pcur = _current(src, "PLAN " + tag, ppat)
         for n in sorted(pol):
             t = pol[n]
-            p = 1 if 1 in t and 0 in t and t[1] < (1 - margin) * t[0] else 0
+            c = pcur.get(n, (0,))[0]
+            o = 1 - c
+            p = o if o in t and c in t and t[o] < (1 - margin) * t[c] else c
             lines.append(f"{dt} plan {n}: policy {p} (0: {t.get(0, 0):.1f} us, 1: {t.get(1, 0):.1f} us)")
             if p:
                 prow.append(f"            case {n}: return 1;")

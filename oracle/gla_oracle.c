@@ -49,8 +49,10 @@
  * over a subdivision, interior faces cancel, and srfScl reduces to 1 / V_T
  * for cells of one shape), so it must equal the corresponding sum of
  * equal-size model-3 tensors; with equal sizes it reduces to model 3
- * itself.  The srfScl factors of cells of different shape have no such
- * check ("parity unpinned" beyond the table itself).  The reference's hcubature values of the regular face
+ * itself.  For a target cell twice as long on one axis only, srfScl's factors
+ * differ per component (1/2 of the subcell sum on xx, yy, zz, yz, the whole
+ * sum on xy, xz, from the way srfScl indexes its table), and the same
+ * subdivision test checks that pattern.  The reference's hcubature values of the regular face
  * pairs (third-party, tolerance 1e-6, src/glaGen.jl:574-576) cannot be run
  * here: model 3 integrates those pairs to their limit instead, so its
  * agreement with the reference is bounded by that tolerance.
