@@ -54,6 +54,27 @@ struct Part {
 };
 
 
+// (i2, i3, c) of row `pen` of a workgroup whose first row is `row0`, rows
+// ordered (i2 fastest, then i3, then c) over n2 x n3 x 3: the divisions are
+// the workgroup's own (uniform, scalar), each lane then steps on by pen rows
+__device__ __forceinline__ void row_coords(int row0, int pen, int n2, int n3, int& i2, int& i3, int& c)
+{
+    const unsigned r0 = (unsigned)row0, u2 = (unsigned)n2, u3 = (unsigned)n3;
+    int a = (int)(r0 % u2), q = (int)(r0 / u2);
+    int b = (int)((unsigned)q % u3), cc = (int)((unsigned)q / u3);
+    a += pen;
+    while (a >= n2) {
+        a -= n2;
+        if (++b == n3) {
+            b = 0;
+            ++cc;
+        }
+    }
+    i2 = a;
+    i3 = b;
+    c = cc;
+}
+
 // Copies the length-N root table (tab[0, N)) and split phases (tab[N, 2N))
 // of an axis into LDS for the workgroup's nt threads.
 template <typename R, int N>
@@ -194,7 +215,8 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
     const int rows = S2 * S3 * 3;
     const int row = blockIdx.x * PEN + pen;
     const bool act = row < rows;
-    const int i2 = act ? row % S2 : 0, i3 = act ? (row / S2) % S3 : 0, c = act ? row / (S2 * S3) : 0;
+    int i2, i3, c;
+    row_coords(blockIdx.x * PEN, act ? pen : 0, S2, S3, i2, i3, c);
     // the row of source partition xp (the whole volume's row without
     // partitions: xp.f = S, xp.d = 1, xp.o = 0)
     const cpx<R>* xr = X + (int64_t)blockIdx.y * xs +
@@ -258,7 +280,8 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     const int rows = T2 * T3 * 3;
     const int row = blockIdx.x * PEN + pen;
     const bool act = row < rows;
-    const int i2 = act ? row % T2 : 0, i3 = act ? (row / T2) % T3 : 0, c = act ? row / (T2 * T3) : 0;
+    int i2, i3, c;
+    row_coords(blockIdx.x * PEN, act ? pen : 0, T2, T3, i2, i3, c);
     const cpx<R>* wr = W1 + (int64_t)blockIdx.y * ws + (((int64_t)c * Z + i3) * Y1 + i2) * (2 * N);
     // the row of target partition yp (the whole volume's row without
     // partitions); the Lippmann-Schwinger form (axpby 2) is for self
