@@ -10,8 +10,10 @@ Prints one JSON line.  Fields beyond the driver's contract:
   roofline        dominant kernel: algorithmic bytes per launch / its average
                   duration between HIP events at its boundaries inside the
                   timed region, against the 8 TB/s HBM peak
-  passes          every kernel's duration (the same events) and algorithmic
-                  bytes
+  passes          every kernel's duration (the same events), algorithmic
+                  bytes, and its rate as a fraction of copy_gbs_same_gpu
+  copy_gbs_same_gpu  a plain 1 GiB device-to-device copy in the same run
+                  (bytes read + written per second)
   cpu_baseline    the CPU oracle (a C restatement of the reference's host
                   algorithm) timed on this host on a bounded sample
   parity          relative L2 of both GPU forms against the FP64 oracle
@@ -406,6 +408,27 @@ def serve_leg(gila, a, log):
             "sweep": sweep}
 
 
+def copy_gbs(torch, nbytes=1 << 30, reps=20):
+    """Streaming bandwidth of a plain device-to-device copy of nbytes on this
+    GPU in the same run (bytes read + written per second), the practical
+    ceiling a bandwidth-bound pass is compared with beside the 8 TB/s peak."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e6)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -523,6 +546,10 @@ def main():
         hts.append(time.perf_counter() - t0)
     host_ms = 1000.0 * float(np.median(hts))
     r32, r64 = res[np.complex64], res[np.complex128]
+    cgbs = copy_gbs(torch)
+    for r in (r32, r64):
+        for p in r["prof"]:
+            p["frac_of_copy"] = p["gbs"] / cgbs
     tot_bytes = sum(p["bytes"] for p in r32["prof"])
     dom = max(r32["prof"], key=lambda p: p["ms"])
     out = {
@@ -547,6 +574,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["gbs"] / HBM_PEAK_GBS, "traffic": None},
         "passes": r32["prof"],
+        "copy_gbs_same_gpu": cgbs,
         "ms_per_step_with_pass_events": r32["ms_ev"],
         "fp64": {"value": 1000.0 / r64["ms"], "unit": "matvecs/s", "ms_per_step": r64["ms"],
                  "hbm_gbs_whole_matvec": sum(p["bytes"] for p in r64["prof"]) / (r64["ms"] * 1e6),
