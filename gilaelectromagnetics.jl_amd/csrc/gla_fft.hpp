@@ -282,6 +282,12 @@ __device__ __forceinline__ void dft(cpx<R>* a, const cpx<R>* __restrict__ tw)
 // with them; ComplexF32 pass 3 159 -> 169 us (fewer instructions, but
 // longer dependent chains at the same occupancy), so off there.
 // ---------------------------------------------------------------------------
+// stages with fewer butterflies than threads deal the (pencil, butterfly)
+// tasks of K pencils over all threads (PencilFft::stage_lds_spread); 0 keeps
+// threads t < NB doing every pencil of butterfly t
+#ifndef GLA_SPREAD
+#define GLA_SPREAD 1
+#endif
 #ifndef GLA_FUSED_TW_F
 #define GLA_FUSED_TW_F 0
 #endif
@@ -438,6 +444,10 @@ struct PencilFft {
         constexpr int NB = N / RAD;                      // butterflies
         constexpr int NU = (NB + TP - 1) / TP;          // per thread (upper bound)
         constexpr int NSP = [] { int ns = 1; for (int i = 0; i < S; ++i) ns *= P.rad[i]; return ns; }();
+        if constexpr (GLA_SPREAD != 0 && K > 1 && NB < TP) {
+            stage_lds_spread<RAD, NB, NSP>(lds, t, pen, tw);
+            return;
+        }
         cpx<R> a[K][NU][RAD];
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
@@ -476,6 +486,52 @@ struct PencilFft {
 #pragma unroll
                     for (int q = 0; q < RAD; ++q) lds[k * KS + slot(base + q * NSP) * NS + pen * PS] = a[k][u][q];
                 }
+            }
+        }
+        lds_sync<SYNC>();
+    }
+
+    // A stage with fewer butterflies than threads per pencil (NB < TP) and
+    // K pencils: the K * NB (pencil, butterfly) tasks are dealt over all TP
+    // threads, task t + TP i to thread t, instead of threads t < NB taking
+    // all K pencils of butterfly t.  The threads of a pencil span several
+    // waves (one per SIMD), so this spreads the stage's arithmetic over the
+    // SIMDs; each task reads its own twiddles.
+    template <int RAD, int NB, int NSP>
+    static __device__ __forceinline__ void stage_lds_spread(cpx<R>* lds, int t, int pen, const cpx<R>* tw)
+    {
+        constexpr int NT = K * NB;
+        constexpr int NUT = (NT + TP - 1) / TP;
+        cpx<R> a[NUT][RAD];
+#pragma unroll
+        for (int i = 0; i < NUT; ++i) {
+            const int tau = t + TP * i;
+            if (NT % TP == 0 || tau < NT) {
+                const int k = tau / NB, j = tau % NB;
+#pragma unroll
+                for (int q = 0; q < RAD; ++q) a[i][q] = lds[k * KS + slot(j + q * NB) * NS + pen * PS];
+            }
+        }
+        lds_sync<SYNC>();
+#pragma unroll
+        for (int i = 0; i < NUT; ++i) {
+            const int tau = t + TP * i;
+            if (NT % TP == 0 || tau < NT) {
+                const int k = tau / NB, j = tau % NB;
+                const int kk = j % NSP;
+                const int base = (j / NSP) * NSP * RAD + kk;
+                cpx<R> w[RAD];
+                if constexpr (NSP > 1) {
+#pragma unroll
+                    for (int q = 1; q < RAD; ++q) {
+                        w[q] = tw[q * kk * (N / (NSP * RAD))];
+                        if constexpr (!fused_tw<R, RAD>()) a[i][q] = twid(a[i][q], w[q]);
+                    }
+                }
+                if constexpr (NSP > 1 && fused_tw<R, RAD>()) dft_tw<R, INV, RAD>(a[i], w);
+                else dft<R, INV, RAD, N>(a[i], tw);
+#pragma unroll
+                for (int q = 0; q < RAD; ++q) lds[k * KS + slot(base + q * NSP) * NS + pen * PS] = a[i][q];
             }
         }
         lds_sync<SYNC>();
