@@ -225,8 +225,8 @@ int64_t int_ratio(int64_t an, int64_t ad, int64_t bn, int64_t bd)
 enum { DIMS_FAST = 0, DIMS_EXACT = 1, DIMS_REF = 2 };
 // layout of operators built from geometry.  DIMS_FAST measured against
 // DIMS_EXACT on the 256x34x34-in-256x256x128 pair (B = (256,145,81)):
-// ComplexF32 matvec 0.696 ms on (256,160,96) against 1.037 ms on the exact
-// lengths, ComplexF64 1.424 against 1.858 ms (the 145-point axis-2 passes
+// ComplexF32 matvec 0.700 ms on (256,160,96) against 0.922 ms on the exact
+// lengths, ComplexF64 1.426 against 1.575 ms (the 145-point axis-2 passes
 // dominate).
 #ifndef GLA_CREATE_DIMS
 #define GLA_CREATE_DIMS DIMS_FAST
