@@ -181,12 +181,16 @@ __device__ __forceinline__ void g_multiply(cpx<R> (&w)[3][E], const cpx<R>* __re
     }
 }
 
-// GLA_XK: the two parities of an axis-1 pencil go through the transform
-// stages together (2, two LDS buffers) or one after the other through one
-// buffer (1: half the LDS per workgroup, so more workgroups fit a CU)
-#ifndef GLA_XK
-#define GLA_XK 1
-#endif
+// The two parities of an axis-1 pencil go through the transform stages
+// together (2: two LDS buffers; a stage with fewer butterflies than threads
+// then deals both parities' butterflies over all lanes) or one after the
+// other through one buffer (1: half the LDS per workgroup, so more
+// workgroups fit a CU) -- per element type and length from xtune.
+template <typename R, int N>
+constexpr int xk_of()
+{
+    return xtune((int)sizeof(cpx<R>), N);
+}
 
 // --------------------------------------------------------------------------
 // Pass 1: x-forward on contiguous rows.  One pencil = one (i2,i3,c) row of X.
@@ -204,7 +208,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
     constexpr int PS = N + (N >> PADSH) + 1;
     // rows are whole pencils inside one wave when TP divides 64
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
-    __shared__ cpx<R> lds[GLA_XK * PEN * PS];
+    __shared__ cpx<R> lds[xk_of<R, N>() * PEN * PS];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PEN);
@@ -234,7 +238,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_fwd_x(const cpx<R>* __restri
         v[0][e] = a + b;
         v[1][e] = phz[n] * (a - b);
     }
-    if constexpr (GLA_XK == 2) {
+    if constexpr (xk_of<R, N>() == 2) {
         PencilFft<R, N, TP, 1, PS, false, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
     } else {
         PencilFft<R, N, TP, 1, PS, false, XSYNC, 1, 0, PADSH>::run(v[0], lds, t, pen, tw);
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
     constexpr int PADSH = 3;
     constexpr int PS = N + (N >> PADSH) + 1;
     constexpr int XSYNC = (64 % TP == 0) ? 1 : 0;
-    __shared__ cpx<R> lds[GLA_XK * PEN * PS];
+    __shared__ cpx<R> lds[xk_of<R, N>() * PEN * PS];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PEN);
@@ -298,7 +302,7 @@ __global__ __launch_bounds__(TP * PEN, MINW) void k_inv_x(const cpx<R>* __restri
         v[0][e] = act ? wr[k] : cpx<R>{R(0), R(0)};
         v[1][e] = act ? wr[N + k] : cpx<R>{R(0), R(0)};
     }
-    if constexpr (GLA_XK == 2) {
+    if constexpr (xk_of<R, N>() == 2) {
         PencilFft<R, N, TP, 1, PS, true, XSYNC, 2, PEN * PS, PADSH>::run(v, lds, t, pen, tw);
     } else {
         PencilFft<R, N, TP, 1, PS, true, XSYNC, 1, 0, PADSH>::run(v[0], lds, t, pen, tw);

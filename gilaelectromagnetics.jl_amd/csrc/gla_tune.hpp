@@ -9,6 +9,8 @@
 //          workgroups fill the GPU better there)
 //   rtune: passes 1, 2, 4, 5 -- elements per thread of the axis-1 rows (ex)
 //          and of the axis-2 columns (ey), and the axis-2 bundle width (pxy)
+//   xtune: passes 1 and 5 -- the two parities of a row through the stages
+//          together (2) or one after the other through half the LDS (1)
 //   ptune: the radix plan of every pencil transform of that length -- 0 takes
 //          the largest radices (fewest LDS exchanges), 1 only radices that
 //          divide the elements per thread (every thread busy in every stage,
@@ -16,6 +18,7 @@
 // Tuning builds force one value for all lengths:
 //   -DGLA_ZT_E_F=.. -DGLA_ZT_PX_F=.. / -DGLA_ZT_E_D=.. -DGLA_ZT_PX_D=..
 //   -DGLA_RT_EX_F=.. -DGLA_RT_EY_F=.. -DGLA_RT_PXY_F=.. (and _D)
+//   -DGLA_XK_F=.. / -DGLA_XK_D=..
 //   -DGLA_PT_F=.. / -DGLA_PT_D=..
 // (_F: ComplexF32, _D: ComplexF64.)
 #pragma once
@@ -135,6 +138,43 @@ constexpr RShape rtune(int complex_bytes, int n)
             case 48: return RShape{8, 4, 8};
         // END TUNED ROWS C128
         default: return n <= 128 ? RShape{4, 4, 8} : RShape{8, 8, 8};
+    }
+#endif
+}
+
+constexpr int xtune(int complex_bytes, int n)
+{
+    if (complex_bytes == 8) {
+#if defined(GLA_XK_F)
+        (void)n;
+        return GLA_XK_F;
+#else
+        switch (n) {
+            // BEGIN TUNED XK C64
+            case 24: return 2;
+            case 80: return 2;
+            case 96: return 2;
+            case 192: return 2;
+            case 320: return 2;
+            case 384: return 2;
+            // END TUNED XK C64
+            default: return 1;
+        }
+#endif
+    }
+#if defined(GLA_XK_D)
+    (void)n;
+    return GLA_XK_D;
+#else
+    switch (n) {
+        // BEGIN TUNED XK C128
+            case 80: return 2;
+            case 96: return 2;
+            case 192: return 2;
+            case 320: return 2;
+            case 384: return 2;
+        // END TUNED XK C128
+        default: return 1;
     }
 #endif
 }

@@ -5,7 +5,7 @@ element type and grid shape; writes the tables of csrc/gla_tune.hpp.
     python tune.py measure --out DIR > res.jsonl  # on the GPU
     python tune.py table   res.jsonl              # rewrites csrc/gla_tune.hpp
 
-Three parts, each a set of candidate libraries that force one choice for every
+Four parts, each a set of candidate libraries that force one choice for every
 length (compile-time overrides of gla_tune.hpp):
 
   zpass  pass 3: (elements per thread, columns per workgroup), timed on a
@@ -18,6 +18,8 @@ length (compile-time overrides of gla_tune.hpp):
   plan   the radix plan policy (every thread busy in every stage, against the
          fewest LDS exchanges) for all passes of a length, timed as pass 3 on
          the large cross-section plus the row passes on the rows grid
+  xk     passes 1 and 5: the two parities of a row through the stages
+         together or one after the other, timed on the rows grid
 
 Operators come from all-zero Green data (the timing does not depend on the
 values), so no setup runs.  Each (part, element type, length[, class]) keeps
@@ -65,12 +67,16 @@ def candidates():
                     {"part": "rows", "c64": f, "c128": d}))
     out.append(("p1", "-DGLA_PT_F=1 -DGLA_PT_D=1", {"part": "plan", "c64": 1, "c128": 1}))
     out.append(("p0", "-DGLA_PT_F=0 -DGLA_PT_D=0", {"part": "plan", "c64": 0, "c128": 0}))
+    out.append(("x1", "-DGLA_XK_F=1 -DGLA_XK_D=1", {"part": "xk", "c64": 1, "c128": 1}))
+    out.append(("x2", "-DGLA_XK_F=2 -DGLA_XK_D=2", {"part": "xk", "c64": 2, "c128": 2}))
     return out
 
 
-def build(out, jobs):
+def build(out, jobs, names=()):
     os.makedirs(out, exist_ok=True)
     for name, extra, _ in candidates():
+        if names and name not in names:
+            continue
         cmd = ["make", "-s", f"-j{jobs}", "-C", CSRC, f"BUILD={out}/obj_{name}", f"LIB={out}/lib_{name}.so",
                f"EXTRA={extra}"]
         if subprocess.run(cmd).returncode:
@@ -118,7 +124,7 @@ def measure_one(name, lengths):
                 rec["large_us"] = timed((128, 128, n), typ)
             if part == "zpass":
                 rec["small_us"] = timed((64, 64, n), typ)
-            if part in ("rows", "plan"):
+            if part in ("rows", "plan", "xk"):
                 rec["rows_us"] = timed((n, n, min(n, 128)), typ)
             print(json.dumps(rec), flush=True)
 
@@ -177,10 +183,12 @@ def table(results, margin):
             small[n] = best if ref not in by or by[best] < (1 - margin) * by[ref] else ref
             lines.append(f"{dt} zpass small {n}: {small[n]} {by[small[n]]:.1f} us (large-grid shape {ref} "
                          f"{by.get(ref, 0):.1f})")
-        src = _sub(src, tag, [f"            case {n}: return {{{s[0]}, {s[1]}}};" for n, s in sorted(large.items())
-                              if s != ZDEFAULT[dt]])
-        src = _sub(src, tag + " SMALL", [f"            case {n}: return {{{s[0]}, {s[1]}}};"
-                                         for n, s in sorted(small.items()) if s != large[n]])
+        # a part without records in this results file keeps its table
+        if zl:
+            src = _sub(src, tag, [f"            case {n}: return {{{s[0]}, {s[1]}}};"
+                                  for n, s in sorted(large.items()) if s != ZDEFAULT[dt]])
+            src = _sub(src, tag + " SMALL", [f"            case {n}: return {{{s[0]}, {s[1]}}};"
+                                             for n, s in sorted(small.items()) if s != large[n]])
         # rows: axis-1 passes (1 + 5) by ex, axis-2 passes (2 + 4) by (ey, pxy)
         rx, ry = {}, {}
         for r in recs:
@@ -203,7 +211,26 @@ def table(results, margin):
                          f"current {d}")
             if (ex, ey[0], ey[1]) != rdefault(dt, n):
                 rows.append(f"            case {n}: return RShape{{{ex}, {ey[0]}, {ey[1]}}};")
-        src = _sub(src, "ROWS " + tag, rows)
+        if rx:
+            src = _sub(src, "ROWS " + tag, rows)
+        # passes 1 and 5: parities together (2) or in turn (1)
+        xk = {}
+        for r in recs:
+            if r["part"] != "xk" or r["dtype"] != dt:
+                continue
+            xk.setdefault(r["n"], {})[r["choice"]] = r["rows_us"][0] + r["rows_us"][4]
+        xrow = []
+        xcur = _current(src, "XK " + tag, ppat)
+        for n in sorted(xk):
+            d = xcur.get(n, (1,))[0]
+            by = xk[n]
+            best = min(by, key=by.get)
+            c = best if d not in by or by[best] < (1 - margin) * by[d] else d
+            lines.append(f"{dt} xk {n}: {c} ({by[c]:.1f} us passes 1+5; current {d} {by.get(d, 0):.1f})")
+            if c != 1:
+                xrow.append(f"            case {n}: return {c};")
+        if xk:
+            src = _sub(src, "XK " + tag, xrow)
         # plan policy: pass 3 on the large grid plus the row passes
         pol = {}
         for r in recs:
@@ -220,7 +247,8 @@ def table(results, margin):
             lines.append(f"{dt} plan {n}: policy {p} (0: {t.get(0, 0):.1f} us, 1: {t.get(1, 0):.1f} us)")
             if p:
                 prow.append(f"            case {n}: return 1;")
-        src = _sub(src, "PLAN " + tag, prow)
+        if pol:
+            src = _sub(src, "PLAN " + tag, prow)
     open(os.path.join(CSRC, "gla_tune.hpp"), "w").write(src)
     print("\n".join(lines))
 
@@ -231,14 +259,14 @@ def main():
     ap.add_argument("results", nargs="?")
     ap.add_argument("--out", default=os.path.join(HERE, "_tune"))
     ap.add_argument("--cand", default="")
-    ap.add_argument("--only", default="", help="comma-separated candidate names to measure")
+    ap.add_argument("--only", default="", help="comma-separated candidate names to build / measure")
     ap.add_argument("--lengths", default=",".join(map(str, LENGTHS)))
     ap.add_argument("--margin", type=float, default=0.05)
     ap.add_argument("--jobs", type=int, default=8)
     a = ap.parse_args()
     lengths = [int(v) for v in a.lengths.split(",")]
     if a.cmd == "build":
-        build(a.out, a.jobs)
+        build(a.out, a.jobs, [s for s in a.only.split(",") if s])
     elif a.cmd == "measure":
         measure(a.out, lengths, [s for s in a.only.split(",") if s])
     elif a.cmd == "_one":
