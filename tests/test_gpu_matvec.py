@@ -304,6 +304,23 @@ def test_every_tuned_pass3_length(gila, orc, gpu, n, typ):
     assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(N, N, (1, 1, 1), fur, x)) <= TOL[typ]
 
 
+ROW_LENGTHS = [16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", ROW_LENGTHS)
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_every_tuned_row_length(gila, orc, gpu, n, typ):
+    # passes 1/5 run a length-n row transform with the shape and parity
+    # batching the tables pick for (element type, n), passes 2/4 a length-n
+    # column transform: every compiled length >= 16 on axis 1 and on axis 2
+    for N in ((n, 6, 4), (4, n, 6)):
+        print(f"case row length {n} N={N} {np.dtype(typ).name}", flush=True)
+        mem, fur = _mem_self(gila, orc, N, typ)
+        x = orc.complex_gaussian(N + (3,), seed=n + 1)
+        assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(N, N, (1, 1, 1), fur, x)) <= TOL[typ]
+
+
 SQUARE_SELF = [(32, 32, 8), (64, 64, 6), (96, 96, 4), (128, 128, 2)]
 SQUARE_EXT = [((32, 40, 4), (32, 24, 4)), ((32, 24, 4), (32, 40, 4)), ((64, 70, 3), (64, 58, 5))]
 
