@@ -283,10 +283,18 @@ __device__ __forceinline__ void dft(cpx<R>* a, const cpx<R>* __restrict__ tw)
 // longer dependent chains at the same occupancy), so off there.
 // ---------------------------------------------------------------------------
 // stages with fewer butterflies than threads deal the (pencil, butterfly)
-// tasks of K pencils over all threads (PencilFft::stage_lds_spread); 0 keeps
-// threads t < NB doing every pencil of butterfly t
-#ifndef GLA_SPREAD
-#define GLA_SPREAD 1
+// tasks of K pencils over all threads (PencilFft::stage_lds_spread); 2 also
+// deals stages whose butterflies do not divide evenly over the threads when
+// that lowers the most tasks one thread runs; 0 keeps threads t < NB doing
+// every pencil of butterfly t.  Per element
+// type: mode 2 measured faster in ComplexF64 (pass 3 at 96^3 165 -> 160 us,
+// 192^3 1326 -> 1288 us) and not in ComplexF32 (the external 256x34x34 in
+// 256x256x128 pair's pass 3 363 -> 379 us).
+#ifndef GLA_SPREAD_F
+#define GLA_SPREAD_F 1
+#endif
+#ifndef GLA_SPREAD_D
+#define GLA_SPREAD_D 2
 #endif
 #ifndef GLA_FUSED_TW_F
 #define GLA_FUSED_TW_F 0
@@ -444,7 +452,10 @@ struct PencilFft {
         constexpr int NB = N / RAD;                      // butterflies
         constexpr int NU = (NB + TP - 1) / TP;          // per thread (upper bound)
         constexpr int NSP = [] { int ns = 1; for (int i = 0; i < S; ++i) ns *= P.rad[i]; return ns; }();
-        if constexpr (GLA_SPREAD != 0 && K > 1 && NB < TP) {
+        constexpr int SPR = sizeof(R) == 8 ? GLA_SPREAD_D : GLA_SPREAD_F;
+        // mode 2: whenever dealing lowers the most tasks one thread runs
+        constexpr bool LOWER = (K * NB + TP - 1) / TP < K * ((NB + TP - 1) / TP);
+        if constexpr (SPR != 0 && K > 1 && (NB < TP || (SPR == 2 && LOWER))) {
             stage_lds_spread<RAD, NB, NSP>(lds, t, pen, tw);
             return;
         }
@@ -491,12 +502,13 @@ struct PencilFft {
         lds_sync<SYNC>();
     }
 
-    // A stage with fewer butterflies than threads per pencil (NB < TP) and
-    // K pencils: the K * NB (pencil, butterfly) tasks are dealt over all TP
-    // threads, task t + TP i to thread t, instead of threads t < NB taking
-    // all K pencils of butterfly t.  The threads of a pencil span several
-    // waves (one per SIMD), so this spreads the stage's arithmetic over the
-    // SIMDs; each task reads its own twiddles.
+    // A stage with fewer butterflies than threads per pencil (NB < TP), or
+    // (spread mode 2) butterflies that do not divide evenly over the threads,
+    // and K pencils: the K * NB (pencil, butterfly) tasks are dealt over all
+    // TP threads, task t + TP i to thread t, instead of thread t taking all K
+    // pencils of butterflies t, t + TP, ...  Inside one wave this fills lanes
+    // that would idle; across waves it evens out their work; each task reads
+    // its own twiddles.
     template <int RAD, int NB, int NSP>
     static __device__ __forceinline__ void stage_lds_spread(cpx<R>* lds, int t, int pen, const cpx<R>* tw)
     {

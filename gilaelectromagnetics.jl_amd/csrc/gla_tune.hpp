@@ -41,7 +41,7 @@ constexpr ZShape ztune_large(int complex_bytes, int n)
         switch (n) {
             // BEGIN TUNED C64
             case 16: return {4, 16};
-            case 24: return {4, 16};
+            case 24: return {2, 8};
             case 32: return {4, 8};
             case 40: return {4, 16};
             case 48: return {4, 16};
@@ -51,7 +51,7 @@ constexpr ZShape ztune_large(int complex_bytes, int n)
             case 128: return {4, 8};
             case 160: return {4, 4};
             case 192: return {8, 8};
-            case 256: return {8, 8};
+            case 256: return {4, 8};
             case 320: return {4, 8};
             case 384: return {4, 8};
             case 512: return {8, 8};
@@ -67,9 +67,12 @@ constexpr ZShape ztune_large(int complex_bytes, int n)
     switch (n) {
         // BEGIN TUNED C128
             case 16: return {4, 16};
+            case 24: return {2, 8};
+            case 40: return {2, 8};
+            case 96: return {2, 4};
             case 160: return {4, 4};
             case 192: return {4, 4};
-            case 256: return {8, 4};
+            case 256: return {4, 4};
             case 320: return {4, 4};
             case 384: return {4, 4};
             case 512: return {8, 4};
@@ -84,18 +87,15 @@ constexpr ZShape ztune_small(int complex_bytes, int n)
     if (complex_bytes == 8) {
         switch (n) {
             // BEGIN TUNED C64 SMALL
-            case 20: return {4, 16};
-            case 64: return {8, 8};
             case 192: return {4, 4};
+            case 384: return {4, 4};
             // END TUNED C64 SMALL
             default: return ztune_large(complex_bytes, n);
         }
     }
     switch (n) {
         // BEGIN TUNED C128 SMALL
-            case 20: return {4, 16};
-            case 40: return {4, 16};
-            case 48: return {4, 16};
+            case 48: return {2, 4};
             case 64: return {4, 4};
         // END TUNED C128 SMALL
         default: return ztune_large(complex_bytes, n);
@@ -136,6 +136,7 @@ constexpr RShape rtune(int complex_bytes, int n)
     switch (n) {
         // BEGIN TUNED ROWS C128
             case 48: return RShape{8, 4, 8};
+            case 256: return RShape{8, 4, 16};
         // END TUNED ROWS C128
         default: return n <= 128 ? RShape{4, 4, 8} : RShape{8, 8, 8};
     }
