@@ -36,13 +36,8 @@
 // phases in LDS, bit 1 pipelines the G^ loads one point ahead, bit 2 issues
 // the first point's G^ loads before the forward transform
 // (measured at 128^3: FP32 215 -> 189-196 us with 7, FP64 403 -> 355-358 us
-// with 3; bit 2 costs FP64 registers)
-#ifndef GLA_ZGG_F
-#define GLA_ZGG_F 7
-#endif
-#ifndef GLA_ZGG_D
-#define GLA_ZGG_D 3
-#endif
+// with 3; bit 2 costs FP64 registers).  Per element type and length from
+// gtune; GLA_ZGG_F / GLA_ZGG_D force one value in tuning builds.
 #ifndef GLA_ZPADSH
 #define GLA_ZPADSH 3
 #endif
@@ -92,7 +87,6 @@ struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
     static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
     static constexpr int ZMINW = DBL ? GLA_ZMINW_D : GLA_ZMINW_F;
-    static constexpr int ZGG = DBL ? GLA_ZGG_D : GLA_ZGG_F;
 };
 
 // pass-3 shape of length N for the large-grid (SMALL = 0) or small-grid
@@ -117,6 +111,7 @@ struct ZK {
     }();
     static constexpr int KB_Z =
         (RMAX_Z <= 16 && 3 * PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024) ? 3 : 1;
+    static constexpr int GG = gtune(CB, N);
 };
 
 template <typename R, int N>
@@ -201,9 +196,9 @@ static hipError_t launch_zpass_shape(const Geom& g, const DevTab& tb, void* W2, 
 #define GLA_ZARGS                                                                                 \
     (cpx<R>*)W2, g.L[0], g.L[1], g.S[2], g.T[2], g.Z, g.w2s, (const cpx<R>*)tb.gf, tb.am[0], tb.am[1], \
         tb.am[2], (const cpx<R>*)tb.phz[2], (const cpx<R>*)tb.tw[2], tb.zorder, tb.zper_xcd, nrhs
-    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
-    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
-    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, Cfg<R>::ZGG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    if (mode == 0) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 0, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, K::GG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    else if (mode == 1) hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 1, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, K::GG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
+    else hipLaunchKernelGGL((k_zpass<R, N, K::TP_Z, PX, 2, Cfg<R>::ZMINW, K::KB_Z, SC, K::WL_Z, K::GG, GLA_ZPADSH>), grid, blk, 0, st, GLA_ZARGS);
 #undef GLA_ZARGS
     return hipGetLastError();
 }

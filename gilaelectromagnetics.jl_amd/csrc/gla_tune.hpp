@@ -11,6 +11,9 @@
 //          and of the axis-2 columns (ey), and the axis-2 bundle width (pxy)
 //   xtune: passes 1 and 5 -- the two parities of a row through the stages
 //          together (2) or one after the other through half the LDS (1)
+//   gtune: pass 3's G^ loads -- 3: root and phase tables in LDS and the G^
+//          loads pipelined one point ahead; 7: also the first point's loads
+//          issued before the forward transform (more registers)
 //   ptune: the radix plan of every pencil transform of that length -- 0 takes
 //          the largest radices (fewest LDS exchanges), 1 only radices that
 //          divide the elements per thread (every thread busy in every stage,
@@ -19,6 +22,7 @@
 //   -DGLA_ZT_E_F=.. -DGLA_ZT_PX_F=.. / -DGLA_ZT_E_D=.. -DGLA_ZT_PX_D=..
 //   -DGLA_RT_EX_F=.. -DGLA_RT_EY_F=.. -DGLA_RT_PXY_F=.. (and _D)
 //   -DGLA_XK_F=.. / -DGLA_XK_D=..
+//   -DGLA_ZGG_F=.. / -DGLA_ZGG_D=..
 //   -DGLA_PT_F=.. / -DGLA_PT_D=..
 // (_F: ComplexF32, _D: ComplexF64.)
 #pragma once
@@ -176,6 +180,33 @@ constexpr int xtune(int complex_bytes, int n)
             case 384: return 2;
         // END TUNED XK C128
         default: return 1;
+    }
+#endif
+}
+
+constexpr int gtune(int complex_bytes, int n)
+{
+    if (complex_bytes == 8) {
+#if defined(GLA_ZGG_F)
+        (void)n;
+        return GLA_ZGG_F;
+#else
+        switch (n) {
+            // BEGIN TUNED GG C64
+            // END TUNED GG C64
+            default: return 7;
+        }
+#endif
+    }
+#if defined(GLA_ZGG_D)
+    (void)n;
+    return GLA_ZGG_D;
+#else
+    switch (n) {
+        // BEGIN TUNED GG C128
+            case 320: return 7;
+        // END TUNED GG C128
+        default: return 3;
     }
 #endif
 }

@@ -5,7 +5,7 @@ element type and grid shape; writes the tables of csrc/gla_tune.hpp.
     python tune.py measure --out DIR > res.jsonl  # on the GPU
     python tune.py table   res.jsonl              # rewrites csrc/gla_tune.hpp
 
-Four parts, each a set of candidate libraries that force one choice for every
+Five parts, each a set of candidate libraries that force one choice for every
 length (compile-time overrides of gla_tune.hpp):
 
   zpass  pass 3: (elements per thread, columns per workgroup), timed on a
@@ -20,6 +20,8 @@ length (compile-time overrides of gla_tune.hpp):
          the large cross-section plus the row passes on the rows grid
   xk     passes 1 and 5: the two parities of a row through the stages
          together or one after the other, timed on the rows grid
+  gg     pass 3: the first point's G^ loads issued before the forward
+         transform or not, timed on the large cross-section
 
 Operators come from all-zero Green data (the timing does not depend on the
 values), so no setup runs.  Each (part, element type, length[, class]) keeps
@@ -69,6 +71,8 @@ def candidates():
     out.append(("p0", "-DGLA_PT_F=0 -DGLA_PT_D=0", {"part": "plan", "c64": 0, "c128": 0}))
     out.append(("x1", "-DGLA_XK_F=1 -DGLA_XK_D=1", {"part": "xk", "c64": 1, "c128": 1}))
     out.append(("x2", "-DGLA_XK_F=2 -DGLA_XK_D=2", {"part": "xk", "c64": 2, "c128": 2}))
+    out.append(("g3", "-DGLA_ZGG_F=3 -DGLA_ZGG_D=3", {"part": "gg", "c64": 3, "c128": 3}))
+    out.append(("g7", "-DGLA_ZGG_F=7 -DGLA_ZGG_D=7", {"part": "gg", "c64": 7, "c128": 7}))
     return out
 
 
@@ -120,7 +124,7 @@ def measure_one(name, lengths):
     for n in lengths:
         for typ, dt in ((np.complex64, "c64"), (np.complex128, "c128")):
             rec = {"cand": name, "part": part, "dtype": dt, "n": n, "choice": desc[dt]}
-            if part in ("zpass", "plan"):
+            if part in ("zpass", "plan", "gg"):
                 rec["large_us"] = timed((128, 128, n), typ)
             if part == "zpass":
                 rec["small_us"] = timed((64, 64, n), typ)
@@ -231,6 +235,25 @@ def table(results, margin):
                 xrow.append(f"            case {n}: return {c};")
         if xk:
             src = _sub(src, "XK " + tag, xrow)
+        # pass 3: first-point G^ prefetch (7) or not (3)
+        gg = {}
+        for r in recs:
+            if r["part"] != "gg" or r["dtype"] != dt:
+                continue
+            gg.setdefault(r["n"], {})[r["choice"]] = r["large_us"][2]
+        grow = []
+        gdef = 7 if dt == "c64" else 3
+        gcur = _current(src, "GG " + tag, ppat)
+        for n in sorted(gg):
+            d = gcur.get(n, (gdef,))[0]
+            by = gg[n]
+            best = min(by, key=by.get)
+            c = best if d not in by or by[best] < (1 - margin) * by[d] else d
+            lines.append(f"{dt} gg {n}: {c} ({by[c]:.1f} us pass 3; current {d} {by.get(d, 0):.1f})")
+            if c != gdef:
+                grow.append(f"            case {n}: return {c};")
+        if gg:
+            src = _sub(src, "GG " + tag, grow)
         # plan policy: pass 3 on the large grid plus the row passes
         pol = {}
         for r in recs:
