@@ -29,18 +29,13 @@
 #endif
 // passes 2 and 4: the two parities through the transform together (2, one
 // LDS buffer each) or one after the other through one buffer (1: half the
-// LDS per workgroup, so more workgroups per CU)
-#ifndef GLA_YK
-#define GLA_YK 2
-#endif
-#ifndef GLA_YK_F
-#define GLA_YK_F GLA_YK
-#endif
-#ifndef GLA_YK_D
-#define GLA_YK_D GLA_YK
-#endif
-template <typename R> struct YK { static constexpr int value = GLA_YK_F; };
-template <> struct YK<double> { static constexpr int value = GLA_YK_D; };
+// LDS per workgroup, so more workgroups per CU) -- per element type and
+// length from ytune
+template <typename R, int N>
+constexpr int yk_of()
+{
+    return gla::ytune((int)sizeof(gla::cpx<R>), N);
+}
 
 namespace gla {
 
@@ -339,7 +334,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restric
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[YK<R>::value * PX * N];
+    __shared__ cpx<R> lds[yk_of<R, N>() * PX * N];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PX);
@@ -379,7 +374,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_fwd_y(const cpx<R>* __restric
         v[0][e] = a + b;
         v[1][e] = phz[n] * (a - b);
     }
-    if constexpr (YK<R>::value == 2) {
+    if constexpr (yk_of<R, N>() == 2) {
         PencilFft<R, N, TP, PX, 1, false, 0, 2, PX * N>::run(v, lds, t, xl, tw);
     } else {
         PencilFft<R, N, TP, PX, 1, false>::run(v[0], lds, t, xl, tw);
@@ -404,7 +399,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
                                                    const cpx<R>* __restrict__ tw)
 {
     constexpr int E = N / TP;
-    __shared__ cpx<R> lds[YK<R>::value * PX * N];
+    __shared__ cpx<R> lds[yk_of<R, N>() * PX * N];
     __shared__ cpx<R> tab[GLA_XYTAB ? 2 * N : 1];
     if constexpr (GLA_XYTAB != 0) {
         stage_tables<R, N>(tab, tw, phz, TP * PX);
@@ -438,7 +433,7 @@ __global__ __launch_bounds__(TP * PX, MINW) void k_inv_y(const cpx<R>* __restric
             v[1][e] = act ? *sp(N + TP * e) : cpx<R>{R(0), R(0)};
         }
     }
-    if constexpr (YK<R>::value == 2) {
+    if constexpr (yk_of<R, N>() == 2) {
         PencilFft<R, N, TP, PX, 1, true, 0, 2, PX * N>::run(v, lds, t, xl, tw);
     } else {
         PencilFft<R, N, TP, PX, 1, true>::run(v[0], lds, t, xl, tw);

@@ -5,14 +5,6 @@
 
 // kernel-shape defaults (overridable at compile time for variant timing); the
 // per-length shapes come from the tables of gla_tune.hpp
-// pass-3 schedule: 0 keeps the input column in registers through the even
-// parity, 2 splits both parities first (fewer registers for FP64)
-#ifndef GLA_ZRELOAD_F
-#define GLA_ZRELOAD_F 0
-#endif
-#ifndef GLA_ZRELOAD_D
-#define GLA_ZRELOAD_D 2
-#endif
 #ifndef GLA_ZBATCH_KB
 #define GLA_ZBATCH_KB 64
 #endif
@@ -85,7 +77,6 @@ __host__ __device__ constexpr int pick_e(int N, int target, int maxtp, bool wave
 template <typename R>
 struct Cfg {
     static constexpr bool DBL = sizeof(R) == 8;
-    static constexpr int ZSCHED = DBL ? GLA_ZRELOAD_D : GLA_ZRELOAD_F;
     static constexpr int ZMINW = DBL ? GLA_ZMINW_D : GLA_ZMINW_F;
 };
 
@@ -112,6 +103,7 @@ struct ZK {
     static constexpr int KB_Z =
         (RMAX_Z <= 16 && 3 * PXZ * PS_Z * (int)sizeof(cpx<R>) <= GLA_ZBATCH_KB * 1024) ? 3 : 1;
     static constexpr int GG = gtune(CB, N);
+    static constexpr int RL = ltune(CB, N);
 };
 
 template <typename R, int N>
@@ -125,7 +117,7 @@ struct LenK {
     // buffers and the staged tables stay within GLA_YLDS_KB
     static constexpr int PXY = [] {
         int px = RS.pxy;
-        while (px > 1 && (YK<R>::value * px + (GLA_XYTAB ? 2 : 0)) * N * CB > GLA_YLDS_KB * 1024) px /= 2;
+        while (px > 1 && (yk_of<R, N>() * px + (GLA_XYTAB ? 2 : 0)) * N * CB > GLA_YLDS_KB * 1024) px /= 2;
         return px;
     }();
     static constexpr int TP_Y = N / pick_e(N, RS.ey, 1024 / PXY, false, POL);
@@ -186,7 +178,7 @@ static hipError_t launch_zpass_shape(const Geom& g, const DevTab& tb, void* W2, 
 {
     using K = ZK<R, N, SMALL>;
     constexpr int PX = K::PXZ;
-    constexpr int SC = Cfg<R>::ZSCHED;
+    constexpr int SC = K::RL;
     dim3 grid(8 * tb.zper_xcd * nrhs, 1, 1);   // right-hand side inside the XCD stretch
     dim3 blk(K::TP_Z * PX);
     const int mode = g.S[2] > g.B[2] ? 1 : (g.T[2] > g.B[2] ? 2 : 0);

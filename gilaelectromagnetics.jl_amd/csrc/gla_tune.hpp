@@ -11,6 +11,11 @@
 //          and of the axis-2 columns (ey), and the axis-2 bundle width (pxy)
 //   xtune: passes 1 and 5 -- the two parities of a row through the stages
 //          together (2) or one after the other through half the LDS (1)
+//   ytune: passes 2 and 4 -- the two parities of a column bundle through the
+//          stages together (2) or one after the other through half the LDS (1)
+//   ltune: pass 3's input column -- 0 kept in registers through the even
+//          parity, 1 read again (an L2 hit) for the odd parity, 2 both
+//          parities split first
 //   gtune: pass 3's G^ loads -- 3: root and phase tables in LDS and the G^
 //          loads pipelined one point ahead; 7: also the first point's loads
 //          issued before the forward transform (more registers)
@@ -23,6 +28,8 @@
 //   -DGLA_RT_EX_F=.. -DGLA_RT_EY_F=.. -DGLA_RT_PXY_F=.. (and _D)
 //   -DGLA_XK_F=.. / -DGLA_XK_D=..
 //   -DGLA_ZGG_F=.. / -DGLA_ZGG_D=..
+//   -DGLA_YK_F=.. / -DGLA_YK_D=.. (or -DGLA_YK=.. for both)
+//   -DGLA_ZRELOAD_F=.. / -DGLA_ZRELOAD_D=..
 //   -DGLA_PT_F=.. / -DGLA_PT_D=..
 // (_F: ComplexF32, _D: ComplexF64.)
 #pragma once
@@ -207,6 +214,65 @@ constexpr int gtune(int complex_bytes, int n)
             case 320: return 7;
         // END TUNED GG C128
         default: return 3;
+    }
+#endif
+}
+
+constexpr int ytune(int complex_bytes, int n)
+{
+#if defined(GLA_YK)
+    (void)complex_bytes;
+    (void)n;
+    return GLA_YK;
+#else
+    if (complex_bytes == 8) {
+#if defined(GLA_YK_F)
+        (void)n;
+        return GLA_YK_F;
+#else
+        switch (n) {
+            // BEGIN TUNED YK C64
+            // END TUNED YK C64
+            default: return 2;
+        }
+#endif
+    }
+#if defined(GLA_YK_D)
+    (void)n;
+    return GLA_YK_D;
+#else
+    switch (n) {
+        // BEGIN TUNED YK C128
+            case 512: return 1;
+        // END TUNED YK C128
+        default: return 2;
+    }
+#endif
+#endif
+}
+
+constexpr int ltune(int complex_bytes, int n)
+{
+    if (complex_bytes == 8) {
+#if defined(GLA_ZRELOAD_F)
+        (void)n;
+        return GLA_ZRELOAD_F;
+#else
+        switch (n) {
+            // BEGIN TUNED RELOAD C64
+            // END TUNED RELOAD C64
+            default: return 0;
+        }
+#endif
+    }
+#if defined(GLA_ZRELOAD_D)
+    (void)n;
+    return GLA_ZRELOAD_D;
+#else
+    switch (n) {
+        // BEGIN TUNED RELOAD C128
+        // END TUNED RELOAD C128
+        default: return 2;
     }
 #endif
 }

@@ -5,7 +5,7 @@ element type and grid shape; writes the tables of csrc/gla_tune.hpp.
     python tune.py measure --out DIR > res.jsonl  # on the GPU
     python tune.py table   res.jsonl              # rewrites csrc/gla_tune.hpp
 
-Five parts, each a set of candidate libraries that force one choice for every
+Seven parts, each a set of candidate libraries that force one choice for every
 length (compile-time overrides of gla_tune.hpp):
 
   zpass  pass 3: (elements per thread, columns per workgroup), timed on a
@@ -22,6 +22,10 @@ length (compile-time overrides of gla_tune.hpp):
          together or one after the other, timed on the rows grid
   gg     pass 3: the first point's G^ loads issued before the forward
          transform or not, timed on the large cross-section
+  yk     passes 2 and 4: the two parities of a bundle together or in turn,
+         timed on the rows grid
+  rl     pass 3: the input column kept, read again or split into both
+         parities first, timed on the large cross-section
 
 Operators come from all-zero Green data (the timing does not depend on the
 values), so no setup runs.  Each (part, element type, length[, class]) keeps
@@ -73,6 +77,10 @@ def candidates():
     out.append(("x2", "-DGLA_XK_F=2 -DGLA_XK_D=2", {"part": "xk", "c64": 2, "c128": 2}))
     out.append(("g3", "-DGLA_ZGG_F=3 -DGLA_ZGG_D=3", {"part": "gg", "c64": 3, "c128": 3}))
     out.append(("g7", "-DGLA_ZGG_F=7 -DGLA_ZGG_D=7", {"part": "gg", "c64": 7, "c128": 7}))
+    out.append(("y1", "-DGLA_YK_F=1 -DGLA_YK_D=1", {"part": "yk", "c64": 1, "c128": 1}))
+    out.append(("y2", "-DGLA_YK_F=2 -DGLA_YK_D=2", {"part": "yk", "c64": 2, "c128": 2}))
+    for v in (0, 1, 2):
+        out.append((f"l{v}", f"-DGLA_ZRELOAD_F={v} -DGLA_ZRELOAD_D={v}", {"part": "rl", "c64": v, "c128": v}))
     return out
 
 
@@ -124,11 +132,11 @@ def measure_one(name, lengths):
     for n in lengths:
         for typ, dt in ((np.complex64, "c64"), (np.complex128, "c128")):
             rec = {"cand": name, "part": part, "dtype": dt, "n": n, "choice": desc[dt]}
-            if part in ("zpass", "plan", "gg"):
+            if part in ("zpass", "plan", "gg", "rl"):
                 rec["large_us"] = timed((128, 128, n), typ)
             if part == "zpass":
                 rec["small_us"] = timed((64, 64, n), typ)
-            if part in ("rows", "plan", "xk"):
+            if part in ("rows", "plan", "xk", "yk"):
                 rec["rows_us"] = timed((n, n, min(n, 128)), typ)
             print(json.dumps(rec), flush=True)
 
@@ -254,6 +262,26 @@ def table(results, margin):
                 grow.append(f"            case {n}: return {c};")
         if gg:
             src = _sub(src, "GG " + tag, grow)
+        # one value per length: passes 2 + 4 for yk, pass 3 for rl
+        for part, tname, dflt, cost in (("yk", "YK", 2, lambda r: r["rows_us"][1] + r["rows_us"][3]),
+                                        ("rl", "RELOAD", 0 if dt == "c64" else 2, lambda r: r["large_us"][2])):
+            byn = {}
+            for r in recs:
+                if r["part"] == part and r["dtype"] == dt:
+                    byn.setdefault(r["n"], {})[r["choice"]] = cost(r)
+            if not byn:
+                continue
+            cur = _current(src, f"{tname} {tag}", ppat)
+            rows_ = []
+            for n in sorted(byn):
+                d = cur.get(n, (dflt,))[0]
+                by = byn[n]
+                best = min(by, key=by.get)
+                c = best if d not in by or by[best] < (1 - margin) * by[d] else d
+                lines.append(f"{dt} {part} {n}: {c} ({by[c]:.1f} us; current {d} {by.get(d, 0):.1f})")
+                if c != dflt:
+                    rows_.append(f"            case {n}: return {c};")
+            src = _sub(src, f"{tname} {tag}", rows_)
         # plan policy: pass 3 on the large grid plus the row passes
         pol = {}
         for r in recs:
