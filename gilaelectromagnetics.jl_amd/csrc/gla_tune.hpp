@@ -101,6 +101,7 @@ constexpr ZShape ztune_small(int complex_bytes, int n)
     if (complex_bytes == 8) {
         switch (n) {
             // BEGIN TUNED C64 SMALL
+            case 128: return {4, 8};
             case 192: return {4, 4};
             case 384: return {4, 4};
             // END TUNED C64 SMALL

@@ -167,40 +167,68 @@ def _current(src, tag, pat):
     return out
 
 
+def _decide(by, incumbent, margin):
+    """The fastest choice when it beats the incumbent by more than `margin`
+    (or the incumbent was not measured), else the incumbent."""
+    best = min(by, key=by.get)
+    return best if incumbent not in by or by[best] < (1 - margin) * by[incumbent] else incumbent
+
+
 def table(results, margin):
+    """Rewrite the tables of gla_tune.hpp from a results file.  Lengths with
+    records are decided afresh (the incumbent stays unless beaten by more than
+    `margin`); entries of lengths without records are kept; a part without
+    records leaves its tables untouched; entries equal to the default are not
+    written."""
     recs = [json.loads(line) for line in open(results) if line.strip()]
     lines = []
     src = open(os.path.join(CSRC, "gla_tune.hpp")).read()
     zpat = r"case (\d+): return \{(\d+), (\d+)\};"
     rpat = r"case (\d+): return RShape\{(\d+), (\d+), (\d+)\};"
     ppat = r"case (\d+): return (\d+);"
+
+    def single(dt, part, tname, dflt, cost, what):
+        nonlocal src
+        byn = {}
+        for r in recs:
+            if r["part"] == part and r["dtype"] == dt:
+                byn.setdefault(r["n"], {})[r["choice"]] = cost(r)
+        if not byn:
+            return
+        vals = {n: v[0] for n, v in _current(src, tname, ppat).items()}
+        for n in sorted(byn):
+            d = vals.get(n, dflt)
+            c = _decide(byn[n], d, margin)
+            vals[n] = c
+            lines.append(f"{dt} {part} {n}: {c} ({byn[n][c]:.1f} us {what}; current {d} {byn[n].get(d, 0):.1f})")
+        src = _sub(src, tname, [f"            case {n}: return {v};" for n, v in sorted(vals.items()) if v != dflt])
+
     for dt, tag in (("c64", "C64"), ("c128", "C128")):
-        # pass 3, large class: the fastest shape where it beats the current entry
+        # pass 3 by grid class: the large-grid table, then small-grid entries
+        # where they differ from the large-grid shape
         zl, zs = {}, {}
         for r in recs:
             if r["part"] != "zpass" or r["dtype"] != dt:
                 continue
             zl.setdefault(r["n"], {})[tuple(r["choice"])] = r["large_us"][2]
             zs.setdefault(r["n"], {})[tuple(r["choice"])] = r["small_us"][2]
-        large, small = {}, {}
-        cur = _current(src, tag, zpat)
-        for n, by in zl.items():
-            d = cur.get(n, ZDEFAULT[dt])
-            best = min(by, key=by.get)
-            large[n] = best if d not in by or by[best] < (1 - margin) * by[d] else d
-            lines.append(f"{dt} zpass large {n}: {large[n]} {by[large[n]]:.1f} us (current {d} {by.get(d, 0):.1f})")
-        for n, by in zs.items():
-            ref = large[n]
-            best = min(by, key=by.get)
-            small[n] = best if ref not in by or by[best] < (1 - margin) * by[ref] else ref
-            lines.append(f"{dt} zpass small {n}: {small[n]} {by[small[n]]:.1f} us (large-grid shape {ref} "
-                         f"{by.get(ref, 0):.1f})")
-        # a part without records in this results file keeps its table
         if zl:
+            large = dict(_current(src, tag, zpat))
+            small = dict(_current(src, tag + " SMALL", zpat))
+            for n, by in zl.items():
+                d = large.get(n, ZDEFAULT[dt])
+                large[n] = _decide(by, d, margin)
+                lines.append(f"{dt} zpass large {n}: {large[n]} {by[large[n]]:.1f} us (current {d} {by.get(d, 0):.1f})")
+            for n, by in zs.items():
+                ref = large[n]
+                small[n] = _decide(by, ref, margin)
+                lines.append(f"{dt} zpass small {n}: {small[n]} {by[small[n]]:.1f} us (large-grid shape {ref} "
+                             f"{by.get(ref, 0):.1f})")
             src = _sub(src, tag, [f"            case {n}: return {{{s[0]}, {s[1]}}};"
                                   for n, s in sorted(large.items()) if s != ZDEFAULT[dt]])
             src = _sub(src, tag + " SMALL", [f"            case {n}: return {{{s[0]}, {s[1]}}};"
-                                             for n, s in sorted(small.items()) if s != large[n]])
+                                             for n, s in sorted(small.items())
+                                             if s != large.get(n, ZDEFAULT[dt])])
         # rows: axis-1 passes (1 + 5) by ex, axis-2 passes (2 + 4) by (ey, pxy)
         rx, ry = {}, {}
         for r in recs:
@@ -210,96 +238,25 @@ def table(results, margin):
             t = r["rows_us"]
             rx.setdefault(r["n"], {})[ex] = min(t[0] + t[4], rx.get(r["n"], {}).get(ex, 1e30))
             ry.setdefault(r["n"], {})[(ey, pxy)] = t[1] + t[3]
-        rows = []
-        rcur = _current(src, "ROWS " + tag, rpat)
-        for n in sorted(rx):
-            d = rcur.get(n, rdefault(dt, n))
-            bx = min(rx[n], key=rx[n].get)
-            ex = bx if d[0] not in rx[n] or rx[n][bx] < (1 - margin) * rx[n][d[0]] else d[0]
-            by_ = min(ry[n], key=ry[n].get)
-            dy = (d[1], d[2])
-            ey = by_ if dy not in ry[n] or ry[n][by_] < (1 - margin) * ry[n][dy] else dy
-            lines.append(f"{dt} rows {n}: ex {ex} ({rx[n][ex]:.1f} us), ey/pxy {ey} ({ry[n][ey]:.1f} us); "
-                         f"current {d}")
-            if (ex, ey[0], ey[1]) != rdefault(dt, n):
-                rows.append(f"            case {n}: return RShape{{{ex}, {ey[0]}, {ey[1]}}};")
         if rx:
-            src = _sub(src, "ROWS " + tag, rows)
-        # passes 1 and 5: parities together (2) or in turn (1)
-        xk = {}
-        for r in recs:
-            if r["part"] != "xk" or r["dtype"] != dt:
-                continue
-            xk.setdefault(r["n"], {})[r["choice"]] = r["rows_us"][0] + r["rows_us"][4]
-        xrow = []
-        xcur = _current(src, "XK " + tag, ppat)
-        for n in sorted(xk):
-            d = xcur.get(n, (1,))[0]
-            by = xk[n]
-            best = min(by, key=by.get)
-            c = best if d not in by or by[best] < (1 - margin) * by[d] else d
-            lines.append(f"{dt} xk {n}: {c} ({by[c]:.1f} us passes 1+5; current {d} {by.get(d, 0):.1f})")
-            if c != 1:
-                xrow.append(f"            case {n}: return {c};")
-        if xk:
-            src = _sub(src, "XK " + tag, xrow)
-        # pass 3: first-point G^ prefetch (7) or not (3)
-        gg = {}
-        for r in recs:
-            if r["part"] != "gg" or r["dtype"] != dt:
-                continue
-            gg.setdefault(r["n"], {})[r["choice"]] = r["large_us"][2]
-        grow = []
-        gdef = 7 if dt == "c64" else 3
-        gcur = _current(src, "GG " + tag, ppat)
-        for n in sorted(gg):
-            d = gcur.get(n, (gdef,))[0]
-            by = gg[n]
-            best = min(by, key=by.get)
-            c = best if d not in by or by[best] < (1 - margin) * by[d] else d
-            lines.append(f"{dt} gg {n}: {c} ({by[c]:.1f} us pass 3; current {d} {by.get(d, 0):.1f})")
-            if c != gdef:
-                grow.append(f"            case {n}: return {c};")
-        if gg:
-            src = _sub(src, "GG " + tag, grow)
-        # one value per length: passes 2 + 4 for yk, pass 3 for rl
-        for part, tname, dflt, cost in (("yk", "YK", 2, lambda r: r["rows_us"][1] + r["rows_us"][3]),
-                                        ("rl", "RELOAD", 0 if dt == "c64" else 2, lambda r: r["large_us"][2])):
-            byn = {}
-            for r in recs:
-                if r["part"] == part and r["dtype"] == dt:
-                    byn.setdefault(r["n"], {})[r["choice"]] = cost(r)
-            if not byn:
-                continue
-            cur = _current(src, f"{tname} {tag}", ppat)
-            rows_ = []
-            for n in sorted(byn):
-                d = cur.get(n, (dflt,))[0]
-                by = byn[n]
-                best = min(by, key=by.get)
-                c = best if d not in by or by[best] < (1 - margin) * by[d] else d
-                lines.append(f"{dt} {part} {n}: {c} ({by[c]:.1f} us; current {d} {by.get(d, 0):.1f})")
-                if c != dflt:
-                    rows_.append(f"            case {n}: return {c};")
-            src = _sub(src, f"{tname} {tag}", rows_)
+            rows = dict(_current(src, "ROWS " + tag, rpat))
+            for n in sorted(rx):
+                d = rows.get(n, rdefault(dt, n))
+                ex = _decide(rx[n], d[0], margin)
+                ey = _decide(ry[n], (d[1], d[2]), margin)
+                rows[n] = (ex, ey[0], ey[1])
+                lines.append(f"{dt} rows {n}: ex {ex} ({rx[n][ex]:.1f} us), ey/pxy {ey} ({ry[n][ey]:.1f} us); "
+                             f"current {d}")
+            src = _sub(src, "ROWS " + tag, [f"            case {n}: return RShape{{{v[0]}, {v[1]}, {v[2]}}};"
+                                            for n, v in sorted(rows.items()) if v != rdefault(dt, n)])
+        # one value per length
+        single(dt, "xk", "XK " + tag, 1, lambda r: r["rows_us"][0] + r["rows_us"][4], "passes 1+5")
+        single(dt, "gg", "GG " + tag, 7 if dt == "c64" else 3, lambda r: r["large_us"][2], "pass 3")
+        single(dt, "yk", "YK " + tag, 2, lambda r: r["rows_us"][1] + r["rows_us"][3], "passes 2+4")
+        single(dt, "rl", "RELOAD " + tag, 0 if dt == "c64" else 2, lambda r: r["large_us"][2], "pass 3")
         # plan policy: pass 3 on the large grid plus the row passes
-        pol = {}
-        for r in recs:
-            if r["part"] != "plan" or r["dtype"] != dt:
-                continue
-            pol.setdefault(r["n"], {})[r["choice"]] = r["large_us"][2] + sum(r["rows_us"][i] for i in (0, 1, 3, 4))
-        prow = []
-        pcur = _current(src, "PLAN " + tag, ppat)
-        for n in sorted(pol):
-            t = pol[n]
-            c = pcur.get(n, (0,))[0]
-            o = 1 - c
-            p = o if o in t and c in t and t[o] < (1 - margin) * t[c] else c
-            lines.append(f"{dt} plan {n}: policy {p} (0: {t.get(0, 0):.1f} us, 1: {t.get(1, 0):.1f} us)")
-            if p:
-                prow.append(f"            case {n}: return 1;")
-        if pol:
-            src = _sub(src, "PLAN " + tag, prow)
+        single(dt, "plan", "PLAN " + tag, 0, lambda r: r["large_us"][2] + sum(r["rows_us"][i] for i in (0, 1, 3, 4)),
+               "pass 3 + rows")
     open(os.path.join(CSRC, "gla_tune.hpp"), "w").write(src)
     print("\n".join(lines))
 
