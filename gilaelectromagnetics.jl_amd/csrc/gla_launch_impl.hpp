@@ -5,6 +5,10 @@
 
 // kernel-shape defaults (overridable at compile time for variant timing); the
 // per-length shapes come from the tables of gla_tune.hpp
+// threads per workgroup of passes 1 and 5 (rows of one pencil each)
+#ifndef GLA_XWG
+#define GLA_XWG 256
+#endif
 #ifndef GLA_ZBATCH_KB
 #define GLA_ZBATCH_KB 64
 #endif
@@ -112,7 +116,7 @@ struct LenK {
     static constexpr int POL = ptune(CB, N);
     static constexpr RShape RS = rtune(CB, N);
     static constexpr int TP_ROW = N / pick_e(N, RS.ex, 64, true, POL);
-    static constexpr int PEN = TP_ROW >= 256 ? 1 : 256 / TP_ROW;
+    static constexpr int PEN = TP_ROW >= GLA_XWG ? 1 : GLA_XWG / TP_ROW;
     // axis-2 passes: the bundle width, capped so the two parities' LDS
     // buffers and the staged tables stay within GLA_YLDS_KB
     static constexpr int PXY = [] {
