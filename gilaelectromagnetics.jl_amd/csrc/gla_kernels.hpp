@@ -27,6 +27,8 @@
 #ifndef GLA_XYTAB
 #define GLA_XYTAB 1
 #endif
+namespace gla {
+
 // passes 2 and 4: the two parities through the transform together (2, one
 // LDS buffer each) or one after the other through one buffer (1: half the
 // LDS per workgroup, so more workgroups per CU) -- per element type and
@@ -34,10 +36,8 @@
 template <typename R, int N>
 constexpr int yk_of()
 {
-    return gla::ytune((int)sizeof(gla::cpx<R>), N);
+    return ytune((int)sizeof(cpx<R>), N);
 }
-
-namespace gla {
 
 // A partition of a volume as pass 1 reads it (source) or pass 5 writes it
 // (target): cell (i1, i2, i3) of the partition is cell (i_d * d_d + o_d) of
