@@ -137,6 +137,41 @@ def test_partitioned_from_egoFur_and_entry_points(gila, orc, gpu, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", list(CASES))
+def test_partitioned_batched_rhs(gila, orc, gpu, case):
+    """Three right-hand sides in one launch: every column equals its own
+    single-right-hand-side launch bitwise and the oracle's partition matvec to
+    1e-12 (FP64)."""
+    import torch
+    from gila_amd import _capi
+    print(f"case partition batched rhs {case}", flush=True)
+    vt, vs = _vols(gila, case)
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), vt, vs, setTyp=np.complex128)
+    d = mem._dims
+    S, T = tuple(d.src_cel), tuple(d.trg_cel)
+    sdiv, tdiv = tuple(d.src_div), tuple(d.trg_div)
+    fur = _fur_flat(mem)
+    nx, ny, nr = 3 * int(np.prod(vs.cel)), 3 * int(np.prod(vt.cel)), 3
+    x = np.stack([orc.complex_gaussian(tuple(vs.cel) + (3,), seed=400 + r) for r in range(nr)], axis=4)
+    xd = torch.from_numpy(x.reshape(-1, order="F").copy()).cuda()
+    yd = torch.empty(ny * nr, dtype=torch.complex128, device="cuda")
+    L = gila.lib()
+    st = torch.cuda.current_stream()
+    _capi.check(L.gla_opr_apply(mem._h, ctypes.c_void_p(xd.data_ptr()), xd.numel(), ctypes.c_void_p(yd.data_ptr()),
+                                yd.numel(), nr, ctypes.c_void_p(st.cuda_stream)))
+    for r in range(nr):
+        x1 = xd[r * nx:(r + 1) * nx].clone()
+        y1 = torch.empty(ny, dtype=torch.complex128, device="cuda")
+        _capi.check(L.gla_opr_apply(mem._h, ctypes.c_void_p(x1.data_ptr()), nx, ctypes.c_void_p(y1.data_ptr()),
+                                    ny, 1, ctypes.c_void_p(st.cuda_stream)))
+        torch.cuda.synchronize()
+        assert torch.equal(y1, yd[r * ny:(r + 1) * ny])
+        yh = y1.cpu().numpy().reshape(tuple(vt.cel) + (3,), order="F")
+        ref = orc.partition_matvec(S, T, sdiv, tdiv, fur, x[..., r])
+        assert orc.rel_l2(yh, ref) <= 1e-12
+
+
+@pytest.mark.gpu
 def test_partitioned_volumes_in_contact_are_refused(gila, gpu):
     vt = gila.GlaVol((4, 4, 4), (2 * H, H, H), (0, 0, 0))
     vs = gila.GlaVol((8, 4, 4), (H, H, H), (8 * H, 0, 0))   # touching faces at x = 4h
