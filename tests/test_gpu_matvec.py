@@ -146,6 +146,28 @@ def test_batched_rhs(gila, orc, gpu, typ):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("S,T", [((6, 4, 12), (2, 12, 4)), ((24, 10, 30), (40, 30, 10))])
+@pytest.mark.parametrize("typ", [np.complex128, np.complex64])
+def test_batched_rhs_external(gila, orc, gpu, S, T, typ):
+    # external pairs (source and target strides differ; pass 3 adds spill-over
+    # rows or writes rows beyond B on some axis) on four right-hand sides in
+    # one launch
+    print(f"case ext batched S={S} T={T} {np.dtype(typ).name}", flush=True)
+    crc = orc.crc_ext(S, T, (H, H, H), (0.25, -0.5, 0.75))
+    sym = (0, 0, 0)
+    fur = orc.fur_from_crc(S, T, sym, crc)
+    B, U = orc.dims(S, T, sym)
+    src = gila.GlaVol(S, ((1, 32),) * 3, (0, 0, 0))
+    trg = gila.GlaVol(T, ((1, 32),) * 3, ((1, 4), (1, 2), (3, 4)))
+    mem = gila.GlaOprMem(gila.GlaKerOpt(True), trg, src, egoFur=orc.fur_branches(fur.astype(typ), U), setTyp=typ)
+    nrhs = 4
+    x = orc.complex_gaussian(tuple(S) + (3, nrhs), seed=44)
+    y = _dev_apply(gila, mem, x, nrhs=nrhs)
+    for r in range(nrhs):
+        assert orc.rel_l2(y[..., r], orc.matvec(S, T, sym, fur, x[..., r])) <= TOL[typ]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("typ", [np.complex128, np.complex64])
 def test_full_size_128_cubed(gila, orc, gpu, typ):
     # the 128^3 self operator at full size against the FP64 oracle
