@@ -326,6 +326,35 @@ def test_every_tuned_pass3_length(gila, orc, gpu, n, typ):
     assert orc.rel_l2(_dev_apply(gila, mem, x), orc.matvec(N, N, (1, 1, 1), fur, x)) <= TOL[typ]
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", Z_LENGTHS)
+def test_every_tuned_pass3_length_large_grid(gila, gpu, n):
+    # 128 x 64 x n cells: 256 x 128 columns, the large-grid class of pass 3
+    # (the test above runs the small class).  Too large for the oracle, so the
+    # two precisions, built from geometry and run on their own shapes, check
+    # each other: a wrong shape in either shows as a difference far above the
+    # ComplexF32 rounding.
+    import torch
+    from gila_amd import _capi
+    N = (128, 64, n)
+    print(f"case pass-3 length {n} large grid", flush=True)
+    vol = gila.GlaVol(N, ((1, 32),) * 3)
+    x = np.random.default_rng(n).standard_normal(N + (3, 2)).view(np.complex128)[..., 0]
+    out = {}
+    for typ in (np.complex128, np.complex64):
+        mem = gila.GlaOprMem(gila.GlaKerOpt(True), vol, setTyp=typ)
+        assert tuple(mem._dims.dev_brn) == N
+        xd = torch.from_numpy(np.asfortranarray(x.astype(typ)).ravel(order="F").copy()).cuda()
+        yd = torch.empty_like(xd)
+        _capi.check(gila.lib().gla_opr_apply(mem._h, ctypes.c_void_p(xd.data_ptr()), xd.numel(),
+                                             ctypes.c_void_p(yd.data_ptr()), yd.numel(), 1, None))
+        torch.cuda.synchronize()
+        out[typ] = yd.cpu().numpy().astype(np.complex128)
+        del mem, xd, yd
+    d = np.linalg.norm(out[np.complex64] - out[np.complex128]) / np.linalg.norm(out[np.complex128])
+    assert d <= 2e-5, d
+
+
 ROW_LENGTHS = [16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512]
 
 
